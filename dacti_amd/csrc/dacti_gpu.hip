@@ -1,0 +1,869 @@
+// dacti_gpu.hip -- host runtime and C ABI of the MI355X leaf-update engine.
+//
+// The host side restates Integrator (src/lib/integrator/integrator.hpp) and
+// its integrate() schedule (impl/integrate.hpp:11-87); every per-cell phase
+// runs as a HIP kernel over the level's cluster list (kernels.hpp).  The
+// host keeps only scalars: the LTS counters and times, the per-level
+// cluster sizes and the node count.
+#include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/dacti_gpu.h"
+#include "kernels.hpp"
+
+using namespace dacti;
+
+namespace {
+
+thread_local std::string g_create_error = "";
+
+struct Sizes {
+    int D, NE, NF, NP, NG, NPX, NGX, NCH, NFACE;
+};
+
+Sizes model_sizes(int model, int D) {
+    Sizes s;
+    s.D = D;
+    s.NE = model == MULTI ? D + 4 : D + 2;
+    s.NF = model == MULTI ? s.NE + D + 1 : s.NE;
+    s.NP = model == MULTI ? D + 4 : 0;
+    s.NG = model == EULER ? 0 : (model == NAVIER_STOKES ? D + 1 : D + 3);
+    s.NPX = s.NP > 0 ? s.NP : 1;
+    s.NGX = s.NG > 0 ? s.NG : 1;
+    s.NCH = 1 << D;
+    s.NFACE = 2 * D;
+    return s;
+}
+
+// one device field: element size and number of components per node
+struct Field {
+    void **ptr;
+    size_t esz;
+    int ncomp;
+};
+
+}  // namespace
+
+struct dacti_handle {
+    int model = 0, D = 0;
+    Sizes sz{};
+    MConst mc{};
+    double side_length = 0, max_dt_cfg = 0, max_cfl = 0;
+    Dev d{};
+    int64_t n = 0;  // nodes in the pool (incl. null node 0)
+    double global_dt = 0;
+    int acti_counter[16] = {0};
+    double acti_time[16] = {0};
+    int max_acti_level = 0;
+    int64_t counts[16] = {0};
+    int64_t pending[16] = {0};
+    int64_t n_substeps = 0, n_global_steps = 0, cell_updates = 0;
+    int64_t flux_launches = 0;
+    double flux_ms = 0, substep_ms = 0;
+    bool timing = false;
+    hipStream_t stream = nullptr;
+    int32_t *d_list = nullptr;
+    int32_t *d_off = nullptr;
+    uint8_t *d_rn = nullptr;
+    int32_t *d_nsel = nullptr;
+    void *d_tmp = nullptr;
+    size_t tmp_bytes = 0;
+    unsigned long long *d_hist = nullptr;
+    Status *d_status = nullptr;
+    Status *h_status = nullptr;
+    int32_t *h_nsel = nullptr;
+    unsigned long long *h_hist = nullptr;
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    std::string err;
+};
+
+namespace {
+
+#define HIPCHK(h, call)                                                                  \
+    do {                                                                                 \
+        hipError_t e_ = (call);                                                          \
+        if (e_ != hipSuccess) {                                                          \
+            (h)->err = std::string(#call) + ": " + hipGetErrorString(e_);              \
+            return DACTI_E_HIP;                                                          \
+        }                                                                                \
+    } while (0)
+
+std::vector<Field> fields_of(dacti_handle *h, Dev &d) {
+    const Sizes &s = h->sz;
+    return {
+        {(void **) &d.parent, 4, 1},         {(void **) &d.child_base, 4, 1},
+        {(void **) &d.nbr, 4, s.NFACE},      {(void **) &d.level, 1, 1},
+        {(void **) &d.acti, 1, 1},           {(void **) &d.cluster, 1, 1},
+        {(void **) &d.buffer, 1, 1},         {(void **) &d.flags, 1, 1},
+        {(void **) &d.bc, 1, s.NFACE},       {(void **) &d.mark, 1, 1},
+        {(void **) &d.center, 8, s.D},       {(void **) &d.size, 8, 1},
+        {(void **) &d.t, 8, 1},              {(void **) &d.p, 8, s.NE},
+        {(void **) &d.u, 8, s.NE},           {(void **) &d.rhs, 8, s.NE},
+        {(void **) &d.tot, 8, s.NF},         {(void **) &d.k, 8, s.NPX},
+        {(void **) &d.t_slope, 8, s.NGX},    {(void **) &d.slope, 8, s.D * s.NE},
+        {(void **) &d.grad, 8, s.D * s.NGX}, {(void **) &d.acc, 8, s.NFACE * s.NF},
+        {(void **) &d.error, 8, 1},          {(void **) &d.ctime, 8, 1},
+        {(void **) &d.max_dt, 8, 1},         {(void **) &d.cfl, 8, 1},
+        {(void **) &d.scratch, 8, 1},
+    };
+}
+
+void free_dev(dacti_handle *h, Dev &d) {
+    for (auto &f : fields_of(h, d)) {
+        if (*f.ptr) (void) hipFree(*f.ptr);
+        *f.ptr = nullptr;
+    }
+}
+
+// allocate a pool of capacity C; when `old` is given, copy its first n nodes
+int alloc_pool(dacti_handle *h, int64_t C, Dev *old, int64_t n) {
+    Dev nd{};
+    nd.C = C;
+    std::vector<Field> nf = fields_of(h, nd);
+    std::vector<Field> of;
+    if (old) of = fields_of(h, *old);
+    for (size_t k = 0; k < nf.size(); ++k) {
+        const size_t bytes = nf[k].esz * (size_t) nf[k].ncomp * (size_t) C;
+        hipError_t e = hipMalloc(nf[k].ptr, bytes);
+        if (e != hipSuccess) {
+            h->err = "device allocation of " + std::to_string(bytes) + " bytes failed: " + hipGetErrorString(e);
+            free_dev(h, nd);
+            return DACTI_E_MEMORY;
+        }
+        HIPCHK(h, hipMemsetAsync(*nf[k].ptr, 0, bytes, h->stream));
+        if (old && n > 0) {
+            for (int cmp = 0; cmp < nf[k].ncomp; ++cmp) {
+                char *dst = (char *) *nf[k].ptr + nf[k].esz * (size_t) cmp * (size_t) C;
+                const char *src = (const char *) *of[k].ptr + of[k].esz * (size_t) cmp * (size_t) old->C;
+                HIPCHK(h, hipMemcpyAsync(dst, src, nf[k].esz * (size_t) n, hipMemcpyDeviceToDevice, h->stream));
+            }
+        }
+    }
+    // initialise the per-node bytes that mean "none"
+    HIPCHK(h, hipMemsetAsync(nd.cluster + n, 0xff, (size_t) (C - n), h->stream));
+    HIPCHK(h, hipMemsetAsync(nd.buffer + n, 0xff, (size_t) (C - n), h->stream));
+    HIPCHK(h, hipMemsetAsync(nd.acti + n, 0xff, (size_t) (C - n), h->stream));
+    if (old) {
+        HIPCHK(h, hipStreamSynchronize(h->stream));
+        free_dev(h, *old);
+    }
+    // scratch buffers sized by capacity
+    if (h->d_list) (void) hipFree(h->d_list);
+    if (h->d_off) (void) hipFree(h->d_off);
+    if (h->d_rn) (void) hipFree(h->d_rn);
+    if (h->d_tmp) (void) hipFree(h->d_tmp);
+    h->d_list = nullptr;
+    h->d_off = nullptr;
+    h->d_rn = nullptr;
+    h->d_tmp = nullptr;
+    HIPCHK(h, hipMalloc(&h->d_list, sizeof(int32_t) * (size_t) C));
+    HIPCHK(h, hipMalloc(&h->d_off, sizeof(int32_t) * (size_t) C));
+    HIPCHK(h, hipMalloc(&h->d_rn, (size_t) C));
+    size_t b1 = 0, b2 = 0, b3 = 0;
+    hipcub::CountingInputIterator<int32_t> it(1);
+    struct Always {
+        __host__ __device__ bool operator()(const int32_t &) const { return true; }
+    };
+    HIPCHK(h, hipcub::DeviceSelect::If(nullptr, b1, it, h->d_list, h->d_nsel, (int) C, Always{}, h->stream));
+    HIPCHK(h, hipcub::DeviceScan::ExclusiveSum(nullptr, b2, h->d_rn, h->d_off, (int) C, h->stream));
+    HIPCHK(h, hipcub::DeviceReduce::Max(nullptr, b3, nd.scratch, (double *) nd.tot, (int) C, h->stream));
+    h->tmp_bytes = std::max(b1, std::max(b2, b3)) + 1024;
+    HIPCHK(h, hipMalloc(&h->d_tmp, h->tmp_bytes));
+    h->d = nd;
+    return DACTI_OK;
+}
+
+int ensure_capacity(dacti_handle *h, int64_t need) {
+    if (need <= h->d.C) return DACTI_OK;
+    int64_t C = h->d.C;
+    while (C < need) C = C + C / 2 + 4096;
+    Dev old = h->d;
+    return alloc_pool(h, C, &old, h->n);
+}
+
+inline unsigned blocks(int64_t n, int b = 256) { return (unsigned) ((n + b - 1) / b); }
+
+struct ClusterIs {
+    const int8_t *cluster;
+    int level;
+    __host__ __device__ bool operator()(const int32_t &i) const { return cluster[i] == level; }
+};
+
+// sorted list of the node ids whose cluster is L (acti_clusters[L])
+int build_list(dacti_handle *h, int L, int64_t *nl) {
+    hipcub::CountingInputIterator<int32_t> it(1);
+    size_t bytes = h->tmp_bytes;
+    HIPCHK(h, hipcub::DeviceSelect::If(h->d_tmp, bytes, it, h->d_list, h->d_nsel, (int) (h->n - 1),
+                                       ClusterIs{h->d.cluster, L}, h->stream));
+    HIPCHK(h, hipMemcpyAsync(h->h_nsel, h->d_nsel, sizeof(int32_t), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    *nl = *h->h_nsel;
+    return DACTI_OK;
+}
+
+int read_status(dacti_handle *h) {
+    HIPCHK(h, hipMemcpyAsync(h->h_status, h->d_status, sizeof(Status), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    return DACTI_OK;
+}
+
+int refresh_counts(dacti_handle *h) {
+    HIPCHK(h, hipMemsetAsync(h->d_hist, 0, sizeof(unsigned long long) * 16, h->stream));
+    const int64_t nb = std::min<int64_t>(blocks(h->n, 256), 4096);
+    k_level_hist<<<(unsigned) std::max<int64_t>(nb, 1), 256, 0, h->stream>>>(h->d.cluster, h->n, h->d_hist);
+    HIPCHK(h, hipGetLastError());
+    HIPCHK(h, hipMemcpyAsync(h->h_hist, h->d_hist, sizeof(unsigned long long) * 16, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    for (int l = 0; l < 16; ++l) h->counts[l] = (int64_t) h->h_hist[l];
+    return DACTI_OK;
+}
+
+// ------------------------------------------------------------------
+// per-model dispatch
+template <int MODEL, int D>
+struct Ops {
+    using S = Sz<MODEL, D>;
+
+    static int init(dacti_handle *h) {
+        Dev &d = h->d;
+        k_init_cells<MODEL, D><<<blocks(d.C), 256, 0, h->stream>>>(d, h->mc, h->n);
+        HIPCHK(h, hipGetLastError());
+        size_t bytes = h->tmp_bytes;
+        double *d_max = d.tot;  // free until the first substep
+        HIPCHK(h, hipcub::DeviceReduce::Max(h->d_tmp, bytes, d.scratch, d_max, (int) d.C, h->stream));
+        double gmax = 0;
+        HIPCHK(h, hipMemcpyAsync(&gmax, d_max, sizeof(double), hipMemcpyDeviceToHost, h->stream));
+        HIPCHK(h, hipStreamSynchronize(h->stream));
+        k_neighbors<D><<<blocks(h->n), 256, 0, h->stream>>>(d, h->mc, h->n);
+        HIPCHK(h, hipGetLastError());
+        // every active leaf, in node order
+        struct LeafActive {
+            const uint8_t *flags;
+            __host__ __device__ bool operator()(const int32_t &i) const {
+                return (flags[i] & F_LEAF) && (flags[i] & F_ACTIVE);
+            }
+        };
+        hipcub::CountingInputIterator<int32_t> it(1);
+        bytes = h->tmp_bytes;
+        HIPCHK(h, hipcub::DeviceSelect::If(h->d_tmp, bytes, it, h->d_list, h->d_nsel, (int) (h->n - 1),
+                                           LeafActive{d.flags}, h->stream));
+        HIPCHK(h, hipMemcpyAsync(h->h_nsel, h->d_nsel, sizeof(int32_t), hipMemcpyDeviceToHost, h->stream));
+        HIPCHK(h, hipStreamSynchronize(h->stream));
+        const int nl = *h->h_nsel;
+        if (nl > 0) {
+            k_slope<MODEL, D><<<blocks(nl), 256, 0, h->stream>>>(d, h->mc, h->d_list, nl);
+            HIPCHK(h, hipGetLastError());
+            if constexpr (MODEL == MULTI) {
+                int rc = chain(h, nl);
+                if (rc) return rc;
+            }
+        }
+        // global_dt = std::min(0.9 * max_dt_leaves, integrator.max_dt): integrator.hpp:150
+        h->global_dt = (h->max_dt_cfg < 0.9 * gmax) ? h->max_dt_cfg : 0.9 * gmax;
+        k_assign<<<blocks(h->n), 256, 0, h->stream>>>(d, h->n, h->global_dt, h->max_cfl);
+        HIPCHK(h, hipGetLastError());
+        int rc = refresh_counts(h);
+        if (rc) return rc;
+        h->max_acti_level = 0;
+        for (int l = 0; l < 16; ++l) {
+            h->acti_counter[l] = 0;
+            h->acti_time[l] = 0.0;
+            if (h->counts[l] > 0) h->max_acti_level = std::max(h->max_acti_level, l);
+        }
+        return DACTI_OK;
+    }
+
+    // multiphase interface fields: impl/integrate.hpp:49-54
+    static int chain(dacti_handle *h, int nl) {
+        if constexpr (MODEL == MULTI) {
+            Dev &d = h->d;
+            for (int it = 0; it < 3; ++it) {
+                k_smooth_alpha<MODEL, D><<<blocks(nl), 256, 0, h->stream>>>(d, h->mc, h->d_list, nl);
+                k_store_k<0><<<blocks(nl), 256, 0, h->stream>>>(d, h->d_list, nl, S::IALPHAS);
+            }
+            k_normal<MODEL, D><<<blocks(nl), 256, 0, h->stream>>>(d, h->mc, h->d_list, nl);
+            k_curvature<MODEL, D><<<blocks(nl), 256, 0, h->stream>>>(d, h->mc, h->d_list, nl);
+            k_smooth_curvature<MODEL, D><<<blocks(nl), 256, 0, h->stream>>>(d, h->mc, h->d_list, nl);
+            k_store_k<0><<<blocks(nl), 256, 0, h->stream>>>(d, h->d_list, nl, S::IKAPPA);
+            HIPCHK(h, hipGetLastError());
+        }
+        return DACTI_OK;
+    }
+
+    // one level substep: impl/integrate.hpp:42-64; returns 1 on force_break
+    static int substep(dacti_handle *h, int L, int signal, int *force_break) {
+        const double dt = h->global_dt / (1 << L);
+        int64_t nl64 = 0;
+        int rc = build_list(h, L, &nl64);
+        if (rc) return rc;
+        const int nl = (int) nl64;
+        HIPCHK(h, hipMemsetAsync(h->d_status, 0, sizeof(Status), h->stream));
+        if (nl > 0) {
+            if (h->timing) HIPCHK(h, hipEventRecord(h->ev[0], h->stream));
+            k_flux<MODEL, D><<<blocks(nl), 256, 0, h->stream>>>(h->d, h->mc, h->d_list, nl, L, dt);
+            k_update<MODEL, D><<<blocks(nl), 256, 0, h->stream>>>(h->d, h->mc, h->d_list, nl, dt, signal, h->d_status);
+            HIPCHK(h, hipGetLastError());
+            if (h->timing) HIPCHK(h, hipEventRecord(h->ev[1], h->stream));
+            h->flux_launches++;
+            h->cell_updates += nl;
+            rc = read_status(h);
+            if (rc) return rc;
+            if (h->h_status->nan_flag) {
+                *force_break = 1;
+                return 1;
+            }
+            k_slope<MODEL, D><<<blocks(nl), 256, 0, h->stream>>>(h->d, h->mc, h->d_list, nl);
+            HIPCHK(h, hipGetLastError());
+            rc = chain(h, nl);
+            if (rc) return rc;
+            // adaptive_refinement(L): impl/adaptive_refinement.hpp:6-323
+            k_flag<MODEL, D><<<blocks(nl), 256, 0, h->stream>>>(h->d, h->mc, h->d_list, nl, h->acti_counter[L],
+                                                                h->d_status);
+            HIPCHK(h, hipGetLastError());
+            rc = read_status(h);
+            if (rc) return rc;
+            int list_nl = nl;
+            if (h->h_status->n_marked > 0) {
+                rc = remesh(h, L, nl, &list_nl);
+                if (rc) return rc;
+            }
+            k_reassign<MODEL, D><<<blocks(list_nl), 256, 0, h->stream>>>(h->d, h->mc, h->d_list, list_nl, L, dt,
+                                                                          h->acti_counter[L], h->d_status);
+            HIPCHK(h, hipGetLastError());
+            if (h->timing) {
+                HIPCHK(h, hipEventRecord(h->ev[2], h->stream));
+            }
+            rc = read_status(h);
+            if (rc) return rc;
+            if (h->timing) {
+                float a = 0, b = 0;
+                HIPCHK(h, hipEventElapsedTime(&a, h->ev[0], h->ev[1]));
+                HIPCHK(h, hipEventElapsedTime(&b, h->ev[0], h->ev[2]));
+                h->flux_ms += a;
+                h->substep_ms += b;
+            }
+            if (L > 0) h->pending[L - 1] += h->h_status->lowered;
+        }
+        h->acti_counter[L] = (h->acti_counter[L] + 1) % 2;
+        h->acti_time[L] += h->global_dt / (1 << L);
+        if (h->pending[L] > 0) {
+            k_flush<<<blocks(h->n), 256, 0, h->stream>>>(h->d, h->n, L);
+            HIPCHK(h, hipGetLastError());
+            h->pending[L] = 0;
+        }
+        rc = refresh_counts(h);
+        if (rc) return rc;
+        h->n_substeps++;
+        return 0;
+    }
+
+    // split / unsplit application and neighbour update (:97-262)
+    static int remesh(dacti_handle *h, int L, int nl, int *list_nl) {
+        k_mark2<D><<<blocks(nl), 256, 0, h->stream>>>(h->d, h->d_list, nl, L, h->d_rn, h->d_status);
+        HIPCHK(h, hipGetLastError());
+        int rc = read_status(h);
+        if (rc) return rc;
+        const int n_ready = h->h_status->n_ready_split;
+        const int n_new = h->h_status->n_new_blocks;
+        bool changed = false;
+        if (n_ready > 0) {
+            size_t bytes = h->tmp_bytes;
+            HIPCHK(h, hipcub::DeviceScan::ExclusiveSum(h->d_tmp, bytes, h->d_rn, h->d_off, nl, h->stream));
+            const int64_t need = h->n + (int64_t) S::NCH * n_new;
+            if (need > h->d.C) {
+                // keep the list across the reallocation
+                std::vector<int32_t> keep((size_t) nl), offs((size_t) nl);
+                HIPCHK(h, hipMemcpyAsync(keep.data(), h->d_list, sizeof(int32_t) * nl, hipMemcpyDeviceToHost, h->stream));
+                HIPCHK(h, hipMemcpyAsync(offs.data(), h->d_off, sizeof(int32_t) * nl, hipMemcpyDeviceToHost, h->stream));
+                HIPCHK(h, hipStreamSynchronize(h->stream));
+                rc = ensure_capacity(h, need + need / 4);
+                if (rc) return rc;
+                HIPCHK(h, hipMemcpyAsync(h->d_list, keep.data(), sizeof(int32_t) * nl, hipMemcpyHostToDevice, h->stream));
+                HIPCHK(h, hipMemcpyAsync(h->d_off, offs.data(), sizeof(int32_t) * nl, hipMemcpyHostToDevice, h->stream));
+                HIPCHK(h, hipStreamSynchronize(h->stream));
+            }
+            k_split_slopes<MODEL, D><<<blocks(nl), 256, 0, h->stream>>>(h->d, h->mc, h->d_list, nl);
+            k_split<MODEL, D><<<blocks(nl), 256, 0, h->stream>>>(h->d, h->mc, h->d_list, nl, h->d_off, h->n, L);
+            HIPCHK(h, hipGetLastError());
+            h->n += (int64_t) S::NCH * n_new;
+            changed = true;
+        }
+        k_unsplit<MODEL, D><<<blocks(nl), 256, 0, h->stream>>>(h->d, h->mc, h->d_list, nl, L, h->d_status);
+        k_clear_marks<<<blocks(nl), 256, 0, h->stream>>>(h->d, h->d_list, nl);
+        HIPCHK(h, hipGetLastError());
+        rc = read_status(h);
+        if (rc) return rc;
+        if (h->h_status->n_unsplit > 0) changed = true;
+        if (changed) {
+            k_neighbors<D><<<blocks(h->n), 256, 0, h->stream>>>(h->d, h->mc, h->n);
+            HIPCHK(h, hipGetLastError());
+            int64_t n2 = 0;
+            rc = build_list(h, L, &n2);
+            if (rc) return rc;
+            *list_nl = (int) n2;
+        }
+        return DACTI_OK;
+    }
+};
+
+// garbage collection: domain.hpp:327-356, impl/acti_utils.hpp:58-74
+__global__ void k_gc_keep(Dev d, int64_t n, uint8_t *keep) {
+    const int64_t i = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    keep[i] = (i == 0) ? 1 : ((d.flags[i] & F_GARBAGE) ? 0 : 1);
+}
+
+template <typename T>
+__global__ void k_gc_move(const T *__restrict__ src, T *__restrict__ dst, int64_t n, int64_t Cs, int64_t Cd, int ncomp,
+                          const uint8_t *keep, const int32_t *o2n) {
+    const int64_t i = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n || !keep[i]) return;
+    const int64_t j = o2n[i];
+    for (int c = 0; c < ncomp; ++c) dst[c * Cd + j] = src[c * Cs + i];
+}
+
+__global__ void k_gc_remap(Dev d, int64_t n, int nface, const uint8_t *keep_old, const int32_t *o2n) {
+    const int64_t j = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+    if (j < 1 || j >= n) return;
+    auto m = [&](int32_t o) -> int32_t { return (o > 0 && keep_old[o]) ? o2n[o] : 0; };
+    d.parent[j] = m(d.parent[j]);
+    d.child_base[j] = d.child_base[j] ? m(d.child_base[j]) : 0;
+    for (int f = 0; f < nface; ++f) d.nbr[f * d.C + j] = m(d.nbr[f * d.C + j]);
+    d.cluster[j] = -1;
+}
+
+int gc(dacti_handle *h) {
+    const int64_t n = h->n;
+    uint8_t *keep = nullptr;
+    int32_t *o2n = nullptr;
+    HIPCHK(h, hipMalloc(&keep, (size_t) n));
+    HIPCHK(h, hipMalloc(&o2n, sizeof(int32_t) * (size_t) n));
+    k_gc_keep<<<blocks(n), 256, 0, h->stream>>>(h->d, n, keep);
+    size_t bytes = 0;
+    HIPCHK(h, hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, keep, o2n, (int) n, h->stream));
+    void *tmp = nullptr;
+    HIPCHK(h, hipMalloc(&tmp, bytes));
+    HIPCHK(h, hipcub::DeviceScan::ExclusiveSum(tmp, bytes, keep, o2n, (int) n, h->stream));
+    int32_t last_off = 0;
+    uint8_t last_keep = 0;
+    HIPCHK(h, hipMemcpyAsync(&last_off, o2n + n - 1, 4, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipMemcpyAsync(&last_keep, keep + n - 1, 1, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    const int64_t n_new = last_off + last_keep;
+    Dev nd{};
+    nd.C = h->d.C;
+    std::vector<Field> nf = fields_of(h, nd), of = fields_of(h, h->d);
+    for (size_t k = 0; k < nf.size(); ++k) {
+        const size_t bytes_k = nf[k].esz * (size_t) nf[k].ncomp * (size_t) nd.C;
+        hipError_t e = hipMalloc(nf[k].ptr, bytes_k);
+        if (e != hipSuccess) {
+            h->err = "gc allocation failed";
+            free_dev(h, nd);
+            return DACTI_E_MEMORY;
+        }
+        HIPCHK(h, hipMemsetAsync(*nf[k].ptr, 0, bytes_k, h->stream));
+        switch (nf[k].esz) {
+        case 1:
+            k_gc_move<uint8_t><<<blocks(n), 256, 0, h->stream>>>((const uint8_t *) *of[k].ptr, (uint8_t *) *nf[k].ptr, n,
+                                                                 h->d.C, nd.C, nf[k].ncomp, keep, o2n);
+            break;
+        case 4:
+            k_gc_move<int32_t><<<blocks(n), 256, 0, h->stream>>>((const int32_t *) *of[k].ptr, (int32_t *) *nf[k].ptr, n,
+                                                                 h->d.C, nd.C, nf[k].ncomp, keep, o2n);
+            break;
+        default:
+            k_gc_move<double><<<blocks(n), 256, 0, h->stream>>>((const double *) *of[k].ptr, (double *) *nf[k].ptr, n,
+                                                                h->d.C, nd.C, nf[k].ncomp, keep, o2n);
+        }
+    }
+    HIPCHK(h, hipMemsetAsync(nd.cluster + n_new, 0xff, (size_t) (nd.C - n_new), h->stream));
+    HIPCHK(h, hipMemsetAsync(nd.buffer + n_new, 0xff, (size_t) (nd.C - n_new), h->stream));
+    HIPCHK(h, hipMemsetAsync(nd.acti + n_new, 0xff, (size_t) (nd.C - n_new), h->stream));
+    k_gc_remap<<<blocks(n_new), 256, 0, h->stream>>>(nd, n_new, h->sz.NFACE, keep, o2n);
+    HIPCHK(h, hipGetLastError());
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    free_dev(h, h->d);
+    h->d = nd;
+    h->n = n_new;
+    (void) hipFree(keep);
+    (void) hipFree(o2n);
+    (void) hipFree(tmp);
+    k_assign<<<blocks(h->n), 256, 0, h->stream>>>(h->d, h->n, h->global_dt, h->max_cfl);
+    HIPCHK(h, hipGetLastError());
+    return refresh_counts(h);
+}
+
+template <int MODEL, int D>
+int integrate_impl(dacti_handle *h, int signal, int *force_break) {
+    *force_break = 0;
+    for (int seq = 1; seq <= (1 << 15); ++seq) {
+        for (int l = 0; l < 16; ++l) {
+            if (seq % (1 << l) != 0) break;
+            const int L = 15 - l;
+            if (L > h->max_acti_level && h->counts[L] > 0) {
+                h->max_acti_level = L;
+                h->acti_time[L] = h->acti_time[L - 1];
+            }
+            if (L > h->max_acti_level) continue;
+            const int rc = Ops<MODEL, D>::substep(h, L, signal, force_break);
+            if (rc < 0) return rc;
+            if (rc == 1) return DACTI_OK;
+        }
+    }
+    int64_t nact = 0;
+    for (int l = 0; l < 16; ++l) nact += h->counts[l];
+    h->n_global_steps++;
+    if ((double) nact / (double) (h->n - 1) < 0.2) return gc(h);
+    return DACTI_OK;
+}
+
+template <int MODEL, int D>
+int dispatch_init(dacti_handle *h) {
+    return Ops<MODEL, D>::init(h);
+}
+
+int init_any(dacti_handle *h) {
+    switch (h->model * 4 + h->D) {
+    case EULER * 4 + 2: return dispatch_init<EULER, 2>(h);
+    case EULER * 4 + 3: return dispatch_init<EULER, 3>(h);
+    case NAVIER_STOKES * 4 + 2: return dispatch_init<NAVIER_STOKES, 2>(h);
+    case NAVIER_STOKES * 4 + 3: return dispatch_init<NAVIER_STOKES, 3>(h);
+    case MULTI * 4 + 2: return dispatch_init<MULTI, 2>(h);
+    case MULTI * 4 + 3: return dispatch_init<MULTI, 3>(h);
+    }
+    h->err = "unsupported model/dimension";
+    return DACTI_E_ARG;
+}
+
+int integrate_any(dacti_handle *h, int signal, int *fb) {
+    switch (h->model * 4 + h->D) {
+    case EULER * 4 + 2: return integrate_impl<EULER, 2>(h, signal, fb);
+    case EULER * 4 + 3: return integrate_impl<EULER, 3>(h, signal, fb);
+    case NAVIER_STOKES * 4 + 2: return integrate_impl<NAVIER_STOKES, 2>(h, signal, fb);
+    case NAVIER_STOKES * 4 + 3: return integrate_impl<NAVIER_STOKES, 3>(h, signal, fb);
+    case MULTI * 4 + 2: return integrate_impl<MULTI, 2>(h, signal, fb);
+    case MULTI * 4 + 3: return integrate_impl<MULTI, 3>(h, signal, fb);
+    }
+    h->err = "unsupported model/dimension";
+    return DACTI_E_ARG;
+}
+
+void destroy_handle(dacti_handle *h) {
+    if (!h) return;
+    if (h->stream) (void) hipStreamSynchronize(h->stream);
+    free_dev(h, h->d);
+    for (void *p : {(void *) h->d_list, (void *) h->d_off, (void *) h->d_rn, (void *) h->d_nsel, h->d_tmp,
+                    (void *) h->d_hist, (void *) h->d_status})
+        if (p) (void) hipFree(p);
+    if (h->h_status) (void) hipHostFree(h->h_status);
+    if (h->h_nsel) (void) hipHostFree(h->h_nsel);
+    if (h->h_hist) (void) hipHostFree(h->h_hist);
+    for (auto &e : h->ev)
+        if (e) (void) hipEventDestroy(e);
+    if (h->stream) (void) hipStreamDestroy(h->stream);
+    delete h;
+}
+
+template <typename T>
+int upload(dacti_handle *h, T *dst, const T *src, int64_t n, int ncomp_src_major, int64_t C) {
+    // src is node-major [i * ncomp + c]; dst is component-major [c * C + i]
+    std::vector<T> tmp((size_t) n);
+    for (int c = 0; c < ncomp_src_major; ++c) {
+        for (int64_t i = 0; i < n; ++i) tmp[(size_t) i] = src[(size_t) i * ncomp_src_major + c];
+        HIPCHK(h, hipMemcpy(dst + (size_t) c * C, tmp.data(), sizeof(T) * (size_t) n, hipMemcpyHostToDevice));
+    }
+    return DACTI_OK;
+}
+
+template <typename T>
+int download(dacti_handle *h, T *dst, const T *src, int64_t n, int ncomp, int64_t C) {
+    if (!dst) return DACTI_OK;
+    std::vector<T> tmp((size_t) n);
+    for (int c = 0; c < ncomp; ++c) {
+        HIPCHK(h, hipMemcpy(tmp.data(), src + (size_t) c * C, sizeof(T) * (size_t) n, hipMemcpyDeviceToHost));
+        for (int64_t i = 0; i < n; ++i) dst[(size_t) i * ncomp + c] = tmp[(size_t) i];
+    }
+    return DACTI_OK;
+}
+
+}  // namespace
+
+// ======================================================================
+// C ABI
+// ======================================================================
+extern "C" {
+
+int dacti_model_sizes(int32_t model, int32_t dim, int32_t *out) {
+    if (!out || model < 0 || model > 2 || dim < 2 || dim > 3) return DACTI_E_ARG;
+    const Sizes s = model_sizes(model, dim);
+    out[0] = s.D;
+    out[1] = s.NE;
+    out[2] = s.NF;
+    out[3] = s.NP;
+    out[4] = s.NG;
+    return DACTI_OK;
+}
+
+int dacti_create(const dacti_case *c, const dacti_tree *t, dacti_handle **out) {
+    if (!c || !t || !out) {
+        g_create_error = "dacti_create: null argument";
+        return DACTI_E_ARG;
+    }
+    *out = nullptr;
+    if (c->model < 0 || c->model > 2 || c->dim < 2 || c->dim > 3) {
+        g_create_error = "dacti_create: unsupported model/dimension";
+        return DACTI_E_ARG;
+    }
+    if (t->n_nodes < 2 || !t->parent || !t->child_base || !t->tree_level || !t->flags || !t->bc || !t->center ||
+        !t->size || !t->p) {
+        g_create_error = "dacti_create: incomplete tree";
+        return DACTI_E_ARG;
+    }
+    if (c->n_zones < 0 || c->n_zones > DACTI_MAX_ZONES || c->max_cfl <= 0) {
+        g_create_error = "dacti_create: invalid case parameters";
+        return DACTI_E_ARG;
+    }
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
+        g_create_error = "dacti_create: no HIP device available";
+        return DACTI_E_HIP;
+    }
+    dacti_handle *h = new dacti_handle();
+    h->model = c->model;
+    h->D = c->dim;
+    h->sz = model_sizes(c->model, c->dim);
+    h->side_length = c->side_length;
+    h->max_dt_cfg = c->max_dt;
+    h->max_cfl = c->max_cfl;
+    MConst &m = h->mc;
+    m.gamma = c->gamma;
+    m.ns_mu = c->ns_mu;
+    m.ns_lambda = c->ns_lambda;
+    m.ns_cp = c->ns_cp;
+    m.ns_R = c->ns_R;
+    m.sigma = c->sigma;
+    m.gamma1 = c->gamma1;
+    m.gamma2 = c->gamma2;
+    m.P1 = c->P1;
+    m.P2 = c->P2;
+    m.cp1 = c->cp1;
+    m.cp2 = c->cp2;
+    m.cv1 = c->cv1;
+    m.cv2 = c->cv2;
+    m.mu1 = c->mu1;
+    m.mu2 = c->mu2;
+    m.lambda1 = c->lambda1;
+    m.lambda2 = c->lambda2;
+    m.max_cfl = c->max_cfl;
+    m.min_error = c->min_error;
+    m.max_error = c->max_error;
+    for (int i = 0; i < 8; ++i) m.weights[i] = c->weights[i];
+    for (int i = 0; i < 3; ++i) {
+        m.active_size[i] = c->active_size[i];
+        m.periodic[i] = c->periodic[i];
+    }
+    m.limiter = c->limiter;
+    m.max_refinement_level = c->max_refinement_level;
+    m.n_zones = c->n_zones;
+    for (int f = 0; f < 6; ++f) {
+        m.bc_type[f] = c->bc_type[f];
+        m.bc_fade_in[f] = c->bc_fade_in[f];
+        for (int e = 0; e < 8; ++e) m.bc_p[f][e] = c->bc_p[f][e];
+    }
+    for (int z = 0; z < 8; ++z) {
+        for (int a = 0; a < 3; ++a) {
+            m.zone_x0[z][a] = c->zone_x0[z][a];
+            m.zone_x1[z][a] = c->zone_x1[z][a];
+        }
+        m.zone_level[z] = c->zone_level[z];
+    }
+    // Cell::volume() and CellFace::area() (integrator/cell.hpp:88-90,135-137):
+    // a node at tree level l has size side_length * 2^-l exactly.
+    for (int l = 0; l < 20; ++l) {
+        const double s = std::ldexp(c->side_length, -l);
+        m.vol[l] = std::pow(s, (double) c->dim);
+        m.area[l] = std::pow(s, (double) (c->dim - 1));
+    }
+    int rc = DACTI_OK;
+    auto fail = [&](int code) {
+        g_create_error = h->err;
+        destroy_handle(h);
+        return code;
+    };
+    if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
+        h->err = "stream creation failed";
+        return fail(DACTI_E_HIP);
+    }
+    for (auto &e : h->ev)
+        if (hipEventCreate(&e) != hipSuccess) {
+            h->err = "event creation failed";
+            return fail(DACTI_E_HIP);
+        }
+    if (hipMalloc(&h->d_nsel, sizeof(int32_t)) != hipSuccess ||
+        hipMalloc(&h->d_hist, sizeof(unsigned long long) * 16) != hipSuccess ||
+        hipMalloc(&h->d_status, sizeof(Status)) != hipSuccess ||
+        hipHostMalloc(&h->h_status, sizeof(Status)) != hipSuccess ||
+        hipHostMalloc(&h->h_nsel, sizeof(int32_t)) != hipSuccess ||
+        hipHostMalloc(&h->h_hist, sizeof(unsigned long long) * 16) != hipSuccess) {
+        h->err = "small allocation failed";
+        return fail(DACTI_E_MEMORY);
+    }
+    const int64_t n = t->n_nodes;
+    h->n = n;
+    const int64_t C = n + std::max<int64_t>(n / 2, 65536);
+    rc = alloc_pool(h, C, nullptr, 0);
+    if (rc) return fail(rc);
+    const Sizes &s = h->sz;
+    Dev &d = h->d;
+    if ((rc = upload<int32_t>(h, d.parent, t->parent, n, 1, C)) ||
+        (rc = upload<int32_t>(h, d.child_base, t->child_base, n, 1, C)) ||
+        (rc = upload<int8_t>(h, d.level, t->tree_level, n, 1, C)) ||
+        (rc = upload<uint8_t>(h, d.flags, t->flags, n, 1, C)) ||
+        (rc = upload<uint8_t>(h, d.bc, t->bc, n, s.NFACE, C)) ||
+        (rc = upload<double>(h, d.center, t->center, n, s.D, C)) ||
+        (rc = upload<double>(h, d.size, t->size, n, 1, C)) || (rc = upload<double>(h, d.p, t->p, n, s.NE, C)))
+        return fail(rc);
+    if (hipMemset(d.cluster, 0xff, (size_t) C) != hipSuccess || hipMemset(d.buffer, 0xff, (size_t) C) != hipSuccess ||
+        hipMemset(d.acti, 0xff, (size_t) C) != hipSuccess) {
+        h->err = "memset failed";
+        return fail(DACTI_E_HIP);
+    }
+    std::vector<double> minus1((size_t) n, -1.0);
+    if (hipMemcpy(d.error, minus1.data(), sizeof(double) * (size_t) n, hipMemcpyHostToDevice) != hipSuccess) {
+        h->err = "upload failed";
+        return fail(DACTI_E_HIP);
+    }
+    rc = init_any(h);
+    if (rc) return fail(rc);
+    if (hipStreamSynchronize(h->stream) != hipSuccess) {
+        h->err = "init failed";
+        return fail(DACTI_E_HIP);
+    }
+    *out = h;
+    return DACTI_OK;
+}
+
+int dacti_integrate(dacti_handle *h, int32_t signal, double *global_dt, int32_t *force_break) {
+    if (!h) return DACTI_E_ARG;
+    int fb = 0;
+    const int rc = integrate_any(h, signal, &fb);
+    if (force_break) *force_break = fb;
+    if (global_dt) *global_dt = h->global_dt;
+    return rc;
+}
+
+int dacti_run(dacti_handle *h, int64_t steps, int32_t signal, int64_t *completed) {
+    if (!h || steps < 0) return DACTI_E_ARG;
+    int64_t s = 0;
+    for (; s < steps; ++s) {
+        int fb = 0;
+        const int rc = integrate_any(h, signal, &fb);
+        if (rc) {
+            if (completed) *completed = s;
+            return rc;
+        }
+        if (fb) break;
+    }
+    if (completed) *completed = s;
+    return DACTI_OK;
+}
+
+int dacti_n_active(dacti_handle *h, int64_t *n) {
+    if (!h || !n) return DACTI_E_ARG;
+    int64_t a = 0;
+    for (int l = 0; l < 16; ++l) a += h->counts[l] + h->pending[l];
+    *n = a;
+    return DACTI_OK;
+}
+
+int dacti_n_nodes(dacti_handle *h, int64_t *n) {
+    if (!h || !n) return DACTI_E_ARG;
+    *n = h->n;
+    return DACTI_OK;
+}
+
+int dacti_export_nodes(dacti_handle *h, const dacti_node_view *v) {
+    if (!h || !v) return DACTI_E_ARG;
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    const Sizes &s = h->sz;
+    const Dev &d = h->d;
+    const int64_t n = h->n, C = d.C;
+    int rc;
+    if ((rc = download<int32_t>(h, v->parent, d.parent, n, 1, C)) ||
+        (rc = download<int32_t>(h, v->child_base, d.child_base, n, 1, C)) ||
+        (rc = download<int32_t>(h, v->nbr, d.nbr, n, s.NFACE, C)) ||
+        (rc = download<int8_t>(h, v->tree_level, d.level, n, 1, C)) ||
+        (rc = download<int8_t>(h, v->acti_level, d.acti, n, 1, C)) ||
+        (rc = download<uint8_t>(h, v->flags, d.flags, n, 1, C)) ||
+        (rc = download<double>(h, v->center, d.center, n, s.D, C)) ||
+        (rc = download<double>(h, v->size, d.size, n, 1, C)) || (rc = download<double>(h, v->t, d.t, n, 1, C)) ||
+        (rc = download<double>(h, v->p, d.p, n, s.NE, C)) || (rc = download<double>(h, v->u, d.u, n, s.NE, C)) ||
+        (rc = download<double>(h, v->slope, d.slope, n, s.D * s.NE, C)) ||
+        (rc = download<double>(h, v->rhs, d.rhs, n, s.NE, C)) || (rc = download<double>(h, v->k, d.k, n, s.NPX, C)) ||
+        (rc = download<double>(h, v->grad, d.grad, n, s.D * s.NGX, C)))
+        return rc;
+    if (v->cluster) {
+        std::vector<int8_t> cl((size_t) n), bf((size_t) n);
+        HIPCHK(h, hipMemcpy(cl.data(), d.cluster, (size_t) n, hipMemcpyDeviceToHost));
+        HIPCHK(h, hipMemcpy(bf.data(), d.buffer, (size_t) n, hipMemcpyDeviceToHost));
+        for (int64_t i = 0; i < n; ++i) v->cluster[i] = bf[(size_t) i] >= 0 ? (int8_t) (16 + bf[(size_t) i]) : cl[(size_t) i];
+    }
+    if (v->meta) {
+        std::vector<int8_t> ac((size_t) n);
+        std::vector<double> er((size_t) n), md((size_t) n), cf((size_t) n);
+        HIPCHK(h, hipMemcpy(ac.data(), d.acti, (size_t) n, hipMemcpyDeviceToHost));
+        HIPCHK(h, hipMemcpy(er.data(), d.error, 8 * (size_t) n, hipMemcpyDeviceToHost));
+        HIPCHK(h, hipMemcpy(md.data(), d.max_dt, 8 * (size_t) n, hipMemcpyDeviceToHost));
+        HIPCHK(h, hipMemcpy(cf.data(), d.cfl, 8 * (size_t) n, hipMemcpyDeviceToHost));
+        for (int64_t i = 0; i < n; ++i) {
+            v->meta[i * 4 + 0] = ac[(size_t) i];
+            v->meta[i * 4 + 1] = er[(size_t) i];
+            v->meta[i * 4 + 2] = md[(size_t) i];
+            v->meta[i * 4 + 3] = cf[(size_t) i];
+        }
+    }
+    return DACTI_OK;
+}
+
+int dacti_get_stats(dacti_handle *h, dacti_stats *s) {
+    if (!h || !s) return DACTI_E_ARG;
+    std::memset(s, 0, sizeof *s);
+    s->n_nodes = h->n;
+    int64_t a = 0;
+    for (int l = 0; l < 16; ++l) a += h->counts[l] + h->pending[l];
+    s->n_active = a;
+    s->n_substeps = h->n_substeps;
+    s->n_global_steps = h->n_global_steps;
+    s->cell_updates = h->cell_updates;
+    s->flux_launches = h->flux_launches;
+    s->flux_kernel_ms = h->flux_ms;
+    s->substep_kernel_ms = h->substep_ms;
+    s->global_dt = h->global_dt;
+    s->max_acti_level = h->max_acti_level;
+    return DACTI_OK;
+}
+
+int dacti_set_timing(dacti_handle *h, int32_t on) {
+    if (!h) return DACTI_E_ARG;
+    h->timing = on != 0;
+    return DACTI_OK;
+}
+
+int dacti_destroy(dacti_handle *h) {
+    destroy_handle(h);
+    return DACTI_OK;
+}
+
+const char *dacti_last_error(dacti_handle *h) {
+    if (!h) return g_create_error.c_str();
+    return h->err.c_str();
+}
+
+}  // extern "C"

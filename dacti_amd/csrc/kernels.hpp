@@ -1,0 +1,1173 @@
+// kernels.hpp -- HIP kernels of the octree leaf update for gfx950.
+//
+// The node pool lives in HBM as structure-of-arrays indexed by node id
+// (stride C = capacity): a level-L substep touches only that level's
+// cluster, given as a sorted list of node ids, and every array access of a
+// wavefront is a gather over consecutive ids.  One thread owns one leaf.
+//
+// Flux accumulation is a gather: a cell sums its own faces in a fixed
+// order; fluxes owed to a cell that is not updated in the current substep
+// go to that cell's per-face accumulator acc[face], written by exactly one
+// thread (no atomics), so results do not depend on scheduling.
+#pragma once
+
+#include "model.hpp"
+
+namespace dacti {
+
+enum : uint8_t {
+    F_LEAF = 1, F_ACTIVE = 2, F_GARBAGE = 4, F_CAN_SPLIT = 8, F_CAN_UNSPLIT = 16, F_SHOULD_SPLIT = 32, F_BOUNDARY = 64
+};
+enum : uint8_t { MK_SPLIT = 1, MK_UNSPLIT_CAND = 2, MK_UNSPLIT_CHILD = 4, MK_READY = 8 };
+
+struct Dev {
+    int64_t C;
+    int32_t *parent, *child_base, *nbr;     // nbr[f*C + i]
+    int8_t *level, *acti, *cluster, *buffer;
+    uint8_t *flags, *bc, *mark;             // bc[f*C + i]
+    double *center;                         // center[d*C + i]
+    double *size, *t;
+    double *p, *u, *rhs, *tot;              // [e*C + i]
+    double *k, *t_slope;                    // [e*C + i]
+    double *slope;                          // [(d*NE + e)*C + i]
+    double *grad;                           // [(d*NGX + e)*C + i]
+    double *acc;                            // [(f*NF + e)*C + i]
+    double *error, *ctime, *max_dt, *cfl;
+    double *scratch;                        // C doubles (smoothing / reductions)
+};
+
+// device status block read back by the host after a substep phase
+struct Status {
+    int32_t nan_flag;
+    int32_t n_marked;       // cells with a split or unsplit-candidate mark
+    int32_t n_ready_split;  // split cells passing the 2:1 check
+    int32_t n_new_blocks;   // of those, without a reusable child block
+    int32_t n_unsplit;
+    int32_t kept;
+    int32_t lowered;
+    int32_t dropped;
+    int32_t raised[16];
+};
+
+__device__ __forceinline__ bool is_leaf(const Dev &d, int32_t i) { return d.flags[i] & F_LEAF; }
+
+// byte-wide atomic OR through the enclosing aligned 32-bit word
+__device__ __forceinline__ void atomic_or_u8(uint8_t *addr, uint8_t v) {
+    const uintptr_t a = reinterpret_cast<uintptr_t>(addr);
+    unsigned int *w = reinterpret_cast<unsigned int *>(a & ~uintptr_t(3));
+    atomicOr(w, (unsigned int) v << (8 * (a & 3)));
+}
+
+// Domain::find / _find: integrator/domain.hpp:65-82,407-417
+template <int D>
+__device__ inline int32_t dom_find(const Dev &d, const MConst &c, double *p, int max_level) {
+    for (int i = 0; i < D; ++i) {
+        const double h = 0.5 * c.active_size[i];
+        if (p[i] < -h || p[i] > h) {
+            if (c.periodic[i]) {
+                if (p[i] < -h) p[i] += c.active_size[i];
+                else p[i] -= c.active_size[i];
+            } else {
+                return 0;
+            }
+        }
+    }
+    const double rs = d.size[1];
+    double x[D];
+    for (int i = 0; i < D; ++i) x[i] = 2.0 * p[i] / rs;
+    int32_t idx = 1;
+    for (int guard = 0; guard < 64; ++guard) {
+        if ((d.flags[idx] & F_LEAF) || d.level[idx] == max_level) return idx;
+        int q = 0;
+        for (int i = 0; i < D; ++i)
+            if (x[i] > 0.0) q |= 1 << i;
+        for (int i = 0; i < D; ++i) x[i] = (x[i] - 0.5 * (((q >> i) & 1) ? 1.0 : -1.0)) * 2.0;
+        idx = d.child_base[idx] + q;
+    }
+    return idx;
+}
+
+// Domain::interpolate_p / _t / _k: integrator/domain.hpp:84-148
+template <int MODEL, int D>
+__device__ void interp_p(const Dev &d, int32_t idx, double *out) {
+    using S = Sz<MODEL, D>;
+    if (d.flags[idx] & F_LEAF) {
+        for (int e = 0; e < S::NE; ++e) out[e] = d.p[e * d.C + idx];
+        return;
+    }
+    double f[S::NE];
+    int n = 0;
+    for (int e = 0; e < S::NE; ++e) f[e] = 0.0;
+    const int32_t cb = d.child_base[idx];
+    for (int ch = 0; ch < S::NCH; ++ch) {
+        if (d.flags[cb + ch] & F_ACTIVE) {
+            double g[S::NE];
+            interp_p<MODEL, D>(d, cb + ch, g);
+            for (int e = 0; e < S::NE; ++e) f[e] = f[e] + g[e];
+            n++;
+        }
+    }
+    for (int e = 0; e < S::NE; ++e) out[e] = f[e] / n;
+}
+
+template <int D>
+__device__ double interp_t(const Dev &d, int32_t idx) {
+    if (d.flags[idx] & F_LEAF) return d.t[idx];
+    double t = 0.0;
+    int n = 0;
+    const int32_t cb = d.child_base[idx];
+    for (int ch = 0; ch < (1 << D); ++ch)
+        if (d.flags[cb + ch] & F_ACTIVE) {
+            t += interp_t<D>(d, cb + ch);
+            n++;
+        }
+    return t / n;
+}
+
+template <int D>
+__device__ double interp_k(const Dev &d, int32_t idx, int kk) {
+    if (d.flags[idx] & F_LEAF) return d.k[kk * d.C + idx];
+    double f = 0.0;
+    int n = 0;
+    const int32_t cb = d.child_base[idx];
+    for (int ch = 0; ch < (1 << D); ++ch)
+        if (d.flags[cb + ch] & F_ACTIVE) {
+            f += interp_k<D>(d, cb + ch, kk);
+            n++;
+        }
+    return f / n;
+}
+
+// Active leaves of subtree `root` touching its face (dim, side), visited in
+// depth-first child order.
+template <int D, typename Fn>
+__device__ inline void for_sub_face_leaves(const Dev &d, int32_t root, int dim, int side, Fn &&fn) {
+    int32_t stack[48];
+    int sp = 0;
+    stack[sp++] = root;
+    while (sp > 0) {
+        const int32_t idx = stack[--sp];
+        const uint8_t fl = d.flags[idx];
+        if (fl & F_LEAF) {
+            if ((fl & F_ACTIVE) && !(fl & F_GARBAGE)) fn(idx);
+            continue;
+        }
+        const int32_t cb = d.child_base[idx];
+        for (int ch = (1 << D) - 1; ch >= 0; --ch)
+            if (((ch >> dim) & 1) == side && sp < 48) stack[sp++] = cb + ch;
+    }
+}
+
+// Cell data one side of a face needs (integrator/cell.hpp:8-38)
+template <int MODEL, int D>
+struct Side {
+    using S = Sz<MODEL, D>;
+    double p[S::NE], s[S::NE], r[S::NE];
+    double t;
+    double kap, kn;               // multi: k[kappa], k[ikn + dim]
+    double g[D][S::NGX];          // NS / multi: gradient
+};
+
+template <int MODEL, int D>
+__device__ __forceinline__ void load_side(const Dev &d, int32_t i, int dim, Side<MODEL, D> &s) {
+    using S = Sz<MODEL, D>;
+    const int64_t C = d.C;
+#pragma unroll
+    for (int e = 0; e < S::NE; ++e) {
+        s.p[e] = d.p[e * C + i];
+        s.s[e] = d.slope[(dim * S::NE + e) * C + i];
+        s.r[e] = d.rhs[e * C + i];
+    }
+    s.t = d.t[i];
+    if constexpr (MODEL == MULTI) {
+        s.kap = d.k[S::IKAPPA * C + i];
+        s.kn = d.k[(S::IKN + dim) * C + i];
+    }
+    if constexpr (S::NG > 0) {
+#pragma unroll
+        for (int a = 0; a < D; ++a)
+#pragma unroll
+            for (int e = 0; e < S::NG; ++e) s.g[a][e] = d.grad[(a * S::NGX + e) * C + i];
+    }
+}
+
+// Model::flux of the face (cn | cp): euler.hpp:156-200,
+// navier_stokes.hpp:61-82, multi.hpp:249-302.  fc: face centre along dim;
+// cnc, cpc: cell centres along dim after any periodic shift.
+template <int MODEL, int D>
+__device__ inline void model_flux(const MConst &c, const Side<MODEL, D> &n, double cnc, const Side<MODEL, D> &pz,
+                                  double cpc, double fc, double area, double t0, double t1, int dim, double *out) {
+    using S = Sz<MODEL, D>;
+    constexpr int NE = S::NE, NF = S::NF, IU = S::IU, IP = S::IP;
+    const double dt_n = 0.5 * (t1 + t0) - n.t;
+    const double dt_p = 0.5 * (t1 + t0) - pz.t;
+    const double dxn = fc - cnc, dxp = fc - cpc;
+    double pn[NE], pp[NE], pm[NE], F[NF];
+#pragma unroll
+    for (int e = 0; e < NE; ++e) {
+        pn[e] = n.p[e] + n.s[e] * dxn;
+        pp[e] = pz.p[e] + pz.s[e] * dxp;
+    }
+#pragma unroll
+    for (int e = 0; e < NE; ++e) {
+        pn[e] = pn[e] - n.r[e] * dt_n;
+        pp[e] = pp[e] - pz.r[e] * dt_p;
+    }
+    if constexpr (MODEL == MULTI) {
+        constexpr int IA = S::IA;
+        const double ka = 0.5 * (n.kap + pz.kap);
+        middle_state<MODEL, D>(c, pn, pp, ka, dim, pm);
+        flux_F<MODEL, D, false>(c, pm, 0.0, dim, F);
+        if (pm[IA] > 1e-5 && pm[IA] < 1 - 1e-5) {
+            const double nn = 0.5 * (n.kn + pz.kn);
+            const double uc = 1.0 * fabs(pm[IU + dim]) * nn;
+            double ke = 0.0;
+            for (int i = 0; i < D; ++i) {
+                F[IU + i] += uc * ((1 - pm[IA]) * pm[0] - pm[IA] * pm[1]) * pm[IU + i];
+                ke += 0.5 * pm[IU + i] * pm[IU + i];
+            }
+            F[0] += uc * (1 - pm[IA]) * pm[0];
+            F[1] += uc * (-pm[IA]) * pm[1];
+            F[IP] += uc * ((1 - pm[IA]) * pm[0] - pm[IA] * pm[1]) * ke +
+                     uc * (1 - pm[IA]) * pm[IA] *
+                         (pm[IP] * (1 / (c.gamma1 - 1) - 1 / (c.gamma2 - 1)) + c.gamma1 * c.P1 / (c.gamma1 - 1) -
+                          c.gamma2 * c.P2 / (c.gamma2 - 1));
+            F[IA] += uc * (1 - pm[IA]) * pm[IA];
+        }
+        double du[D][S::NGX], vel[D], fv[D + 1];
+        for (int a = 0; a < D; ++a)
+            for (int e = 0; e < S::NG; ++e) du[a][e] = 0.5 * (n.g[a][e] + pz.g[a][e]);
+        for (int i = 0; i < D; ++i) vel[i] = 0.5 * (n.p[IU + i] + pz.p[IU + i]);
+        visc_Fv<D, S::NGX>(du, vel, model_mu<MODEL, D>(c, pm), dim, fv);
+        for (int i = 0; i <= D; ++i) F[IU + i] -= fv[i];
+        F[IU + D] -= du[dim][S::IDT] * model_lambda<MODEL, D>(c, pm);
+#pragma unroll
+        for (int e = 0; e < NF; ++e) out[e] = F[e] * area * (t1 - t0);
+    } else {
+        middle_state<MODEL, D>(c, pn, pp, 0.0, dim, pm);
+        flux_F<MODEL, D, false>(c, pm, 0.0, dim, F);
+#pragma unroll
+        for (int e = 0; e < NF; ++e) out[e] = F[e] * area * (t1 - t0);
+        if constexpr (MODEL == NAVIER_STOKES) {
+            double du[D][S::NGX], vel[D], fv[D + 1];
+            for (int a = 0; a < D; ++a)
+                for (int e = 0; e < S::NG; ++e) du[a][e] = 0.5 * (n.g[a][e] + pz.g[a][e]);
+            for (int i = 0; i < D; ++i) vel[i] = 0.5 * (n.p[IU + i] + pz.p[IU + i]);
+            visc_Fv<D, S::NGX>(du, vel, c.ns_mu, dim, fv);
+            for (int i = 0; i <= D; ++i) out[IU + i] -= fv[i] * area * (t1 - t0);
+            out[IU + D] -= c.ns_lambda * du[dim][S::IDT] * area * (t1 - t0);
+        }
+    }
+}
+
+// The owner x computes its face (dim, dir): impl/flux.hpp:32-91.  Returns F
+// with the reference's sign convention (x receives -F if dir==1, +F if
+// dir==0; the neighbour the opposite).
+template <int MODEL, int D>
+__device__ inline void owner_face_flux(const Dev &d, const MConst &c, int32_t x, int dim, int dir, double dt,
+                                       double *F) {
+    const int64_t C = d.C;
+    const int32_t nb = d.nbr[(2 * dim + dir) * C + x];
+    const double xc = d.center[dim * C + x];
+    const double xs = d.size[x];
+    const double fc = xc + xs * (dir - 0.5);
+    const double t0 = d.t[x];
+    const double t1 = t0 + dt;
+    const double area = c.area[d.level[x]];
+    double nbc = d.center[dim * C + nb];
+    if (c.periodic[dim]) {
+        if (dir == 1) {
+            if (xc + xs > 0.5 * c.active_size[dim]) nbc += c.active_size[dim];
+        } else {
+            if (xc - xs < -0.5 * c.active_size[dim]) nbc -= c.active_size[dim];
+        }
+    }
+    Side<MODEL, D> sx, sn;
+    load_side<MODEL, D>(d, x, dim, sx);
+    load_side<MODEL, D>(d, nb, dim, sn);
+    if (dir == 1) model_flux<MODEL, D>(c, sx, xc, sn, nbc, fc, area, t0, t1, dim, F);
+    else model_flux<MODEL, D>(c, sn, nbc, sx, xc, fc, area, t0, t1, dim, F);
+}
+
+// boundary_flux: integrator/boundary.hpp:130-160 (already scaled)
+template <int MODEL, int D>
+__device__ inline void boundary_flux(const Dev &d, const MConst &c, int32_t y, int face, int dim, int dir,
+                                     double dt, double *out) {
+    using S = Sz<MODEL, D>;
+    constexpr int NE = S::NE, NF = S::NF, IU = S::IU, IP = S::IP;
+    const int64_t C = d.C;
+    double pc[NE], pb[NE], F[NF];
+    for (int e = 0; e < NE; ++e) pc[e] = d.p[e * C + y];
+    const double t0 = d.t[y], t1 = t0 + dt;
+    boundary_value<MODEL, D>(c, face, pc, t0, pb);
+    const int type = c.bc_type[face];
+    if (type == BC_FREESLIP) {
+        flux_F<MODEL, D, true>(c, pc, pb[IU + dim], dim, F);
+    } else if (type == BC_NOSLIP) {
+        for (int e = 0; e < NF; ++e) F[e] = 0.0;
+        F[IU + dim] = pb[IP];
+    } else {
+        double pm[NE];
+        middle_state<MODEL, D>(c, pb, pc, 0.0, dim, pm);
+        flux_F<MODEL, D, false>(c, pm, 0.0, dim, F);
+    }
+    if constexpr (S::NG > 0) {
+        double g[D][S::NGX], vel0[D], fv[D + 1];
+        for (int a = 0; a < D; ++a)
+            for (int e = 0; e < S::NGX; ++e) g[a][e] = d.grad[(a * S::NGX + e) * C + y];
+        for (int i = 0; i < D; ++i) vel0[i] = 0.0;
+        visc_Fv<D, S::NGX>(g, vel0, model_mu<MODEL, D>(c, pc), dim, fv);
+        for (int i = 0; i <= D; ++i) F[IU + i] -= fv[i];
+    }
+    const double direction = 2 * dir - 1;
+    const double area = c.area[d.level[y]];
+    for (int e = 0; e < NF; ++e) out[e] = F[e] * area * (t1 - t0) * direction;
+}
+
+__device__ __forceinline__ int bc_face(const Dev &d, int32_t y, int f) {
+    const uint8_t b = d.bc[f * d.C + y];
+    return b ? b - 1 : f;
+}
+
+// ------------------------------------------------------------------ flux
+// One level-L substep's flux totals: impl/flux.hpp:10-97 (restated as a
+// gather, see the file comment).  Writes tot[NF] of every listed cell and
+// deposits fluxes owed to cells outside the cluster.
+template <int MODEL, int D>
+__global__ void __launch_bounds__(256) k_flux(Dev d, MConst c, const int32_t *__restrict__ list, int nl, int L,
+                                               double dt) {
+    using S = Sz<MODEL, D>;
+    constexpr int NF = S::NF;
+    const int li = blockIdx.x * blockDim.x + threadIdx.x;
+    if (li >= nl) return;
+    const int64_t C = d.C;
+    const int32_t y = list[li];
+    const int ylev = d.level[y];
+    double tot[NF];
+#pragma unroll
+    for (int e = 0; e < NF; ++e) tot[e] = 0.0;
+    for (int dim = 0; dim < D; ++dim) {
+        for (int dir = 0; dir < 2; ++dir) {
+            const int f = 2 * dim + dir;
+            const int32_t nb = d.nbr[f * C + y];
+            double cc[NF], F[NF];
+#pragma unroll
+            for (int e = 0; e < NF; ++e) cc[e] = 0.0;
+            if (nb == 0) {
+                boundary_flux<MODEL, D>(d, c, y, bc_face(d, y, f), dim, dir, dt, F);
+#pragma unroll
+                for (int e = 0; e < NF; ++e) cc[e] = cc[e] - F[e];
+            } else if (d.flags[nb] & F_LEAF) {
+                const int nlev = d.level[nb];
+                const int nacti = d.acti[nb];
+                const int ncl = d.cluster[nb];
+                if (nlev == ylev && nacti > L) {
+                    // deposits of a same-level neighbour on a finer time level
+#pragma unroll
+                    for (int e = 0; e < NF; ++e) {
+                        cc[e] = d.acc[(f * NF + e) * C + y];
+                        d.acc[(f * NF + e) * C + y] = 0.0;
+                    }
+                }
+                bool y_computes, nb_computes = false;
+                if (nlev < ylev) {
+                    y_computes = nacti <= L;
+                } else {
+                    y_computes = nacti <= L && (dir == 1 || nacti < L);
+                    nb_computes = ncl == L && dir == 0 && d.nbr[(2 * dim + 1) * C + nb] == y;
+                }
+                if (y_computes) {
+                    owner_face_flux<MODEL, D>(d, c, y, dim, dir, dt, F);
+#pragma unroll
+                    for (int e = 0; e < NF; ++e) cc[e] = dir == 1 ? cc[e] - F[e] : cc[e] + F[e];
+                    if (ncl != L) {
+                        const int fo = 2 * dim + (1 - dir);
+                        if (nlev == ylev) {
+#pragma unroll
+                            for (int e = 0; e < NF; ++e) {
+                                double &a = d.acc[(fo * NF + e) * C + nb];
+                                a = dir == 1 ? a + F[e] : a - F[e];
+                            }
+                        } else {
+                            // coarser receiver: the first computing sibling deposits
+                            // all computing siblings' fluxes in sub-face order
+                            const int32_t rn = d.nbr[fo * C + nb];
+                            int32_t first = 0;
+                            for_sub_face_leaves<D>(d, rn, dim, dir, [&](int32_t x) {
+                                if (first == 0 && d.cluster[x] == L && d.nbr[f * C + x] == nb) first = x;
+                            });
+                            if (first == y) {
+                                for_sub_face_leaves<D>(d, rn, dim, dir, [&](int32_t x) {
+                                    if (d.cluster[x] == L && d.nbr[f * C + x] == nb) {
+                                        double G[NF];
+                                        owner_face_flux<MODEL, D>(d, c, x, dim, dir, dt, G);
+                                        for (int e = 0; e < NF; ++e) {
+                                            double &a = d.acc[(fo * NF + e) * C + nb];
+                                            a = dir == 1 ? a + G[e] : a - G[e];
+                                        }
+                                    }
+                                });
+                            }
+                        }
+                    }
+                } else if (nb_computes) {
+                    owner_face_flux<MODEL, D>(d, c, nb, dim, 1, dt, F);
+#pragma unroll
+                    for (int e = 0; e < NF; ++e) cc[e] = cc[e] + F[e];
+                }
+            } else {
+                // finer neighbours: received deposits plus this substep's sub-faces
+                bool any_fast = false;
+                for_sub_face_leaves<D>(d, nb, dim, 1 - dir, [&](int32_t x) {
+                    if (d.acti[x] > L) any_fast = true;
+                });
+                if (any_fast) {
+#pragma unroll
+                    for (int e = 0; e < NF; ++e) {
+                        cc[e] = d.acc[(f * NF + e) * C + y];
+                        d.acc[(f * NF + e) * C + y] = 0.0;
+                    }
+                }
+                for_sub_face_leaves<D>(d, nb, dim, 1 - dir, [&](int32_t x) {
+                    if (d.cluster[x] == L && d.nbr[(2 * dim + (1 - dir)) * C + x] == y) {
+                        double G[NF];
+                        owner_face_flux<MODEL, D>(d, c, x, dim, 1 - dir, dt, G);
+                        for (int e = 0; e < NF; ++e) cc[e] = (1 - dir) == 1 ? cc[e] + G[e] : cc[e] - G[e];
+                    }
+                });
+            }
+#pragma unroll
+            for (int e = 0; e < NF; ++e) tot[e] = tot[e] + cc[e];
+        }
+    }
+#pragma unroll
+    for (int e = 0; e < NF; ++e) d.tot[e * C + y] = tot[e];
+}
+
+// ---------------------------------------------------------------- update
+// cell_update: impl/cell_update.hpp:9-90, multi::source multi.hpp:186-203
+template <int MODEL, int D>
+__global__ void __launch_bounds__(256) k_update(Dev d, MConst c, const int32_t *__restrict__ list, int nl, double dt,
+                                                 int signal, Status *st) {
+    using S = Sz<MODEL, D>;
+    constexpr int NE = S::NE, NF = S::NF, IU = S::IU, IP = S::IP;
+    const int li = blockIdx.x * blockDim.x + threadIdx.x;
+    if (li >= nl) return;
+    const int64_t C = d.C;
+    const int32_t y = list[li];
+    double u[NE], p[NE], du[NF];
+    const double vol = c.vol[d.level[y]];
+#pragma unroll
+    for (int e = 0; e < NE; ++e) {
+        u[e] = d.u[e * C + y];
+        p[e] = d.p[e * C + y];
+    }
+    double gold[S::NGX], gnew[S::NGX];
+    if constexpr (S::NG > 0) grad_input<MODEL, D>(c, p, gold);
+#pragma unroll
+    for (int e = 0; e < NF; ++e) du[e] = d.tot[e * C + y] / vol;
+#pragma unroll
+    for (int e = 0; e < NE; ++e) u[e] += du[e];
+    if constexpr (MODEL == MULTI) {
+        constexpr int IA = S::IA;
+        if (!signal) {
+            if (u[IA] > 0.001)
+                for (int e = IU; e < IU + D; ++e) u[e] = 0.0;
+        }
+        const double kap = d.k[S::IKAPPA * C + y];
+        double s[NE];
+        for (int e = 0; e < NE; ++e) s[e] = 0.0;
+        for (int i = 0; i < D; ++i) s[IU + i] = -c.sigma * kap * du[IA + 2 + i];
+        s[IP] = c.sigma * kap * (-du[IA] + p[IA] * du[IA + 1]);
+        s[IA] = -p[IA] * du[IA + 1];
+        for (int e = 0; e < NE; ++e) u[e] = u[e] + s[e];
+    }
+    prim_from_cons<MODEL, D>(c, u, p);
+    if (isnan(p[IP]) || isnan(p[0])) atomicOr(&st->nan_flag, 1);
+    p[0] = smax(p[0], 1e-3);
+    if constexpr (MODEL == MULTI) p[1] = smax(p[1], 1e-6);
+    p[IP] = smax(p[IP], 1e-3);
+    cons_from_prim<MODEL, D>(c, p, u);
+#pragma unroll
+    for (int e = 0; e < NE; ++e) {
+        d.u[e * C + y] = u[e];
+        d.p[e * C + y] = p[e];
+    }
+    if constexpr (S::NG > 0) {
+        grad_input<MODEL, D>(c, p, gnew);
+        for (int e = 0; e < S::NG; ++e) d.t_slope[e * C + y] = (gnew[e] - gold[e]) / dt;
+    }
+    if constexpr (MODEL == MULTI) {
+        d.k[0 * C + y] = c.mu1 * p[S::IA] + c.mu2 * (1 - p[S::IA]);
+        d.k[S::IALPHAS * C + y] = p[S::IA];
+    }
+    d.t[y] = d.t[y] + dt;
+}
+
+// ----------------------------------------------------------------- slope
+// compute_slope: impl/slopes.hpp:15-76
+template <int MODEL, int D>
+__device__ inline void compute_slope_dev(const Dev &d, const MConst &c, int32_t y) {
+    using S = Sz<MODEL, D>;
+    constexpr int NE = S::NE;
+    const int64_t C = d.C;
+    double pc[NE], slope[D][NE];
+    for (int e = 0; e < NE; ++e) pc[e] = d.p[e * C + y];
+    const double ys = d.size[y], yt = d.t[y];
+    double ts[S::NGX];
+    if constexpr (S::NG > 0)
+        for (int e = 0; e < S::NG; ++e) ts[e] = d.t_slope[e * C + y];
+    for (int dim = 0; dim < D; ++dim) {
+        const int32_t in = d.nbr[(2 * dim) * C + y], ip = d.nbr[(2 * dim + 1) * C + y];
+        double pn[NE], pp[NE], xn, xp, tn, tp;
+        const double xc = 0.0;
+        if (in == 0) {
+            boundary_value<MODEL, D>(c, bc_face(d, y, 2 * dim), pc, yt, pn);
+            xn = -ys;
+            tn = yt;
+        } else {
+            if (!(d.flags[in] & F_LEAF)) {
+                interp_p<MODEL, D>(d, in, pn);
+                tn = interp_t<D>(d, in);
+            } else {
+                for (int e = 0; e < NE; ++e) pn[e] = d.p[e * C + in];
+                tn = d.t[in];
+            }
+            xn = -0.5 * (ys + d.size[in]);
+        }
+        if (ip == 0) {
+            boundary_value<MODEL, D>(c, bc_face(d, y, 2 * dim + 1), pc, yt, pp);
+            xp = ys;
+            tp = yt;
+        } else {
+            if (!(d.flags[ip] & F_LEAF)) {
+                interp_p<MODEL, D>(d, ip, pp);
+                tp = interp_t<D>(d, ip);
+            } else {
+                for (int e = 0; e < NE; ++e) pp[e] = d.p[e * C + ip];
+                tp = d.t[ip];
+            }
+            xp = 0.5 * (ys + d.size[ip]);
+        }
+        for (int e = 0; e < NE; ++e) {
+            const double sn = (pc[e] - pn[e]) / (xc - xn);
+            const double sp = (pp[e] - pc[e]) / (xp - xc);
+            slope[dim][e] = limiter(c.limiter, sn, sp);
+            d.slope[(dim * NE + e) * C + y] = slope[dim][e];
+        }
+        if constexpr (S::NG > 0) {
+            double gp[S::NG], gn[S::NG];
+            grad_input<MODEL, D>(c, pp, gp);
+            grad_input<MODEL, D>(c, pn, gn);
+            for (int e = 0; e < S::NG; ++e)
+                d.grad[(dim * S::NGX + e) * C + y] = (gp[e] - gn[e] - ts[e] * (tp - tn)) / (xp - xn);
+        }
+    }
+    double rhs[NE];
+    const double kap = MODEL == MULTI ? d.k[S::IKAPPA * C + y] : 0.0;
+    compute_rhs<MODEL, D>(c, pc, slope, kap, rhs);
+    for (int e = 0; e < NE; ++e) d.rhs[e * C + y] = rhs[e];
+}
+
+template <int MODEL, int D>
+__global__ void __launch_bounds__(256) k_slope(Dev d, MConst c, const int32_t *__restrict__ list, int nl) {
+    const int li = blockIdx.x * blockDim.x + threadIdx.x;
+    if (li >= nl) return;
+    compute_slope_dev<MODEL, D>(d, c, list[li]);
+}
+
+// ------------------------------------------------- multiphase interface
+template <int MODEL, int D>
+__device__ inline double bc_alpha(const Dev &d, const MConst &c, int32_t y, int f) {
+    using S = Sz<MODEL, D>;
+    double pc[S::NE], pb[S::NE];
+    for (int e = 0; e < S::NE; ++e) pc[e] = d.p[e * d.C + y];
+    boundary_value<MODEL, D>(c, bc_face(d, y, f), pc, d.t[y], pb);
+    return pb[S::IA];
+}
+
+// smooth_alpha, one Jacobi sweep: impl/smooth.hpp:9-60
+template <int MODEL, int D>
+__global__ void k_smooth_alpha(Dev d, MConst c, const int32_t *__restrict__ list, int nl) {
+    using S = Sz<MODEL, D>;
+    const int li = blockIdx.x * blockDim.x + threadIdx.x;
+    if (li >= nl) return;
+    const int64_t C = d.C;
+    const int32_t y = list[li];
+    const double ka = d.k[S::IALPHAS * C + y];
+    double acc = 0.0;
+    for (int dim = 0; dim < D; ++dim) {
+        const int32_t in = d.nbr[(2 * dim) * C + y], ip = d.nbr[(2 * dim + 1) * C + y];
+        double an, ap;
+        if (in == 0) an = bc_alpha<MODEL, D>(d, c, y, 2 * dim);
+        else if (!(d.flags[in] & F_LEAF)) an = interp_k<D>(d, in, S::IALPHAS);
+        else an = d.k[S::IALPHAS * C + in];
+        if (ip == 0) ap = bc_alpha<MODEL, D>(d, c, y, 2 * dim + 1);
+        else if (!(d.flags[ip] & F_LEAF)) ap = interp_k<D>(d, ip, S::IALPHAS);
+        else ap = d.k[S::IALPHAS * C + ip];
+        acc += 0.5 * (an + ap) + ka;
+    }
+    d.scratch[li] = acc / (2 * D);
+}
+
+template <int KK>
+__global__ void k_store_k(Dev d, const int32_t *__restrict__ list, int nl, int kk) {
+    const int li = blockIdx.x * blockDim.x + threadIdx.x;
+    if (li >= nl) return;
+    d.k[kk * d.C + list[li]] = d.scratch[li];
+}
+
+// compute_normal: impl/normal_curvature.hpp:6-55
+template <int MODEL, int D>
+__global__ void k_normal(Dev d, MConst c, const int32_t *__restrict__ list, int nl) {
+    using S = Sz<MODEL, D>;
+    const int li = blockIdx.x * blockDim.x + threadIdx.x;
+    if (li >= nl) return;
+    const int64_t C = d.C;
+    const int32_t y = list[li];
+    const double ys = d.size[y];
+    double n[D];
+    for (int dim = 0; dim < D; ++dim) {
+        const int32_t in = d.nbr[(2 * dim) * C + y], ip = d.nbr[(2 * dim + 1) * C + y];
+        double xn, xp, an, ap;
+        if (in == 0) {
+            an = bc_alpha<MODEL, D>(d, c, y, 2 * dim);
+            xn = -0.5 * ys;
+        } else {
+            an = !(d.flags[in] & F_LEAF) ? interp_k<D>(d, in, S::IALPHAS) : d.k[S::IALPHAS * C + in];
+            xn = -0.5 * (ys + d.size[in]);
+        }
+        if (ip == 0) {
+            ap = bc_alpha<MODEL, D>(d, c, y, 2 * dim + 1);
+            xp = 0.5 * ys;
+        } else {
+            ap = !(d.flags[ip] & F_LEAF) ? interp_k<D>(d, ip, S::IALPHAS) : d.k[S::IALPHAS * C + ip];
+            xp = 0.5 * (ys + d.size[ip]);
+        }
+        n[dim] = (ap - an) / (xp - xn);
+    }
+    double dd = 0;
+    for (int i = 0; i < D; ++i) dd += n[i] * n[i];
+    const double nrm = sqrt(dd);
+    for (int i = 0; i < D; ++i) d.k[(S::IKN + i) * C + y] = (nrm < 1e-10) ? 0.0 : n[i] / nrm;
+}
+
+// compute_curvature: impl/normal_curvature.hpp:58-102
+template <int MODEL, int D>
+__global__ void k_curvature(Dev d, MConst c, const int32_t *__restrict__ list, int nl) {
+    using S = Sz<MODEL, D>;
+    const int li = blockIdx.x * blockDim.x + threadIdx.x;
+    if (li >= nl) return;
+    const int64_t C = d.C;
+    const int32_t y = list[li];
+    const double ys = d.size[y];
+    double kappa = 0.0;
+    for (int dim = 0; dim < D; ++dim) {
+        const int32_t in = d.nbr[(2 * dim) * C + y], ip = d.nbr[(2 * dim + 1) * C + y];
+        double xn, xp, nn, np;
+        if (in == 0) {
+            nn = 0.0;
+            xn = -0.5 * ys;
+        } else {
+            nn = !(d.flags[in] & F_LEAF) ? interp_k<D>(d, in, S::IKN + dim) : d.k[(S::IKN + dim) * C + in];
+            xn = -0.5 * (ys + d.size[in]);
+        }
+        if (ip == 0) {
+            np = 0.0;
+            xp = 0.5 * ys;
+        } else {
+            np = !(d.flags[ip] & F_LEAF) ? interp_k<D>(d, ip, S::IKN + dim) : d.k[(S::IKN + dim) * C + ip];
+            xp = 0.5 * (ys + d.size[ip]);
+        }
+        kappa -= (np - nn) / (xp - xn);
+    }
+    d.k[S::IKAPPA * C + y] = kappa;
+}
+
+// smooth_curvature (one iteration): impl/smooth.hpp:63-142
+template <int MODEL, int D>
+__global__ void k_smooth_curvature(Dev d, MConst c, const int32_t *__restrict__ list, int nl) {
+    using S = Sz<MODEL, D>;
+    const int li = blockIdx.x * blockDim.x + threadIdx.x;
+    if (li >= nl) return;
+    const int64_t C = d.C;
+    const int32_t y = list[li];
+    const double kc = d.k[S::IKAPPA * C + y];
+    const double al = d.k[S::IALPHAS * C + y];
+    const double wa = al * (1 - al);
+    const double wc = wa * wa;
+    double w = 0.0, ks = 0.0;
+    for (int dim = 0; dim < D; ++dim) {
+        const int32_t in = d.nbr[(2 * dim) * C + y], ip = d.nbr[(2 * dim + 1) * C + y];
+        double kn, kp, an, ap;
+        if (in == 0) {
+            kn = 0.0;
+            an = bc_alpha<MODEL, D>(d, c, y, 2 * dim);
+        } else if (!(d.flags[in] & F_LEAF)) {
+            double pi[S::NE];
+            kn = interp_k<D>(d, in, S::IKAPPA);
+            interp_p<MODEL, D>(d, in, pi);
+            an = pi[S::IA];
+        } else {
+            kn = d.k[S::IKAPPA * C + in];
+            an = d.p[S::IA * C + in];
+        }
+        if (ip == 0) {
+            kp = 0.0;
+            ap = bc_alpha<MODEL, D>(d, c, y, 2 * dim + 1);
+        } else if (!(d.flags[ip] & F_LEAF)) {
+            double pi[S::NE];
+            kp = interp_k<D>(d, ip, S::IKAPPA);
+            interp_p<MODEL, D>(d, ip, pi);
+            ap = pi[S::IA];
+        } else {
+            kp = d.k[S::IKAPPA * C + ip];
+            ap = d.p[S::IA * C + ip];
+        }
+        const double xa = an * (1 - an), xb = ap * (1 - ap);
+        const double wn = xa * xa, wp = xb * xb;
+        ks += 0.5 * (wn * kn + wp * kp) + kc * wc;
+        w += 0.5 * (wn + wp) + wc;
+    }
+    d.scratch[li] = (w > 1e-6) ? ks / w : 0.0;
+}
+
+// --------------------------------------------------- refinement flagging
+// Scene::can_split: scene/scene.hpp:50-61
+template <int D>
+__device__ inline bool scene_can_split(const MConst &c, const double *x, int level) {
+    if (level < c.max_refinement_level) return true;
+    for (int z = 0; z < c.n_zones; ++z) {
+        const double x3[3] = {x[0], D > 1 ? x[1 % D] : 0.0, D > 2 ? x[2 % D] : 0.0};
+        if (level < c.zone_level[z] && x3[0] < c.zone_x1[z][0] && x3[1] < c.zone_x1[z][1] && x3[2] < c.zone_x1[z][2] &&
+            x3[0] > c.zone_x0[z][0] && x3[1] > c.zone_x0[z][1] && x3[2] > c.zone_x0[z][2])
+            return true;
+    }
+    return false;
+}
+
+// error estimate + split / unsplit marks: impl/adaptive_refinement.hpp:18-94,
+// weighted_laplacian::estimate error_estimators.hpp:19-37
+template <int MODEL, int D>
+__global__ void __launch_bounds__(256) k_flag(Dev d, MConst c, const int32_t *__restrict__ list, int nl,
+                                               int counter, Status *st) {
+    using S = Sz<MODEL, D>;
+    constexpr int NE = S::NE;
+    const int li = blockIdx.x * blockDim.x + threadIdx.x;
+    if (li >= nl) return;
+    const int64_t C = d.C;
+    const int32_t y = list[li];
+    double pc[NE], ctr[D];
+    for (int e = 0; e < NE; ++e) pc[e] = d.p[e * C + y];
+    for (int i = 0; i < D; ++i) ctr[i] = d.center[i * C + y];
+    const double ys = d.size[y];
+    const int ylev = d.level[y];
+    double err = 0.0;
+    for (int dim = 0; dim < D; ++dim) {
+        double d2[NE];
+        for (int e = 0; e < NE; ++e) d2[e] = pc[e] * -2.0;
+        for (int dir = 0; dir < 2; ++dir) {
+            const int32_t nb = d.nbr[(2 * dim + dir) * C + y];
+            if (nb == 0) {
+                for (int e = 0; e < NE; ++e) d2[e] = 0.0;
+                break;
+            }
+            double q[NE];
+            if (!(d.flags[nb] & F_LEAF)) {
+                interp_p<MODEL, D>(d, nb, q);
+            } else if (d.level[nb] < ylev) {
+                double x[D];
+                for (int i = 0; i < D; ++i) x[i] = ctr[i];
+                x[dim] = ctr[dim] + ys * (2 * dir - 1.0);
+                if (c.periodic[dim] && dir == 0 && x[dim] < -0.5 * c.active_size[dim]) x[dim] += c.active_size[dim];
+                if (c.periodic[dim] && dir == 1 && x[dim] > 0.5 * c.active_size[dim]) x[dim] -= c.active_size[dim];
+                for (int e = 0; e < NE; ++e) q[e] = d.p[e * C + nb];
+                for (int a = 0; a < D; ++a) {
+                    const double dx = x[a] - d.center[a * C + nb];
+                    for (int e = 0; e < NE; ++e) q[e] = q[e] + d.slope[(a * NE + e) * C + nb] * dx;
+                }
+            } else {
+                for (int e = 0; e < NE; ++e) q[e] = d.p[e * C + nb];
+            }
+            for (int e = 0; e < NE; ++e) d2[e] = d2[e] + q[e];
+        }
+        double sq = 0.0;
+        for (int e = 0; e < NE; ++e) {
+            const double v = d2[e] * c.weights[e];
+            sq += v * v;
+        }
+        err += sq;
+    }
+    err = sqrt(err);
+    d.error[y] = err;
+    uint8_t fl = d.flags[y];
+    if (fl & F_CAN_SPLIT) {
+        if (!scene_can_split<D>(c, ctr, ylev)) fl &= ~F_CAN_SPLIT;
+        d.flags[y] = fl;
+    }
+    uint8_t m = 0;
+    if ((err > c.max_error && (fl & F_CAN_SPLIT)) || (fl & F_SHOULD_SPLIT)) m |= MK_SPLIT;
+    if (err < c.min_error && (fl & F_CAN_UNSPLIT) && ylev > 0 && counter == 1) m |= MK_UNSPLIT_CAND;
+    d.mark[y] = m;
+    if (m) atomicAdd(&st->n_marked, 1);
+}
+
+// unsplit detection (:115-149) and split readiness (:154-165)
+template <int D>
+__global__ void k_mark2(Dev d, const int32_t *__restrict__ list, int nl, int L, uint8_t *ready_new,
+                        Status *st) {
+    const int li = blockIdx.x * blockDim.x + threadIdx.x;
+    if (li >= nl) return;
+    const int64_t C = d.C;
+    const int32_t y = list[li];
+    uint8_t m = d.mark[y];
+    uint8_t rn = 0;
+    const int ylev = d.level[y];
+    if ((m & MK_UNSPLIT_CAND) && (d.flags[y] & F_CAN_UNSPLIT)) {
+        bool ok = true;
+        for (int f = 0; f < 2 * D; ++f) {
+            const int32_t nb = d.nbr[f * C + y];
+            if (nb != 0 && (!(d.flags[nb] & F_LEAF) || d.acti[nb] > L)) ok = false;
+        }
+        if (ok) m |= MK_UNSPLIT_CHILD;
+    }
+    if (m & MK_SPLIT) {
+        bool ready = true;
+        for (int f = 0; f < 2 * D; ++f) {
+            const int32_t nb = d.nbr[f * C + y];
+            if (nb != 0 && d.level[nb] < ylev - 1) ready = false;
+        }
+        if (ready) {
+            m |= MK_READY;
+            rn = d.child_base[y] == 0 ? 1 : 0;
+            atomicAdd(&st->n_ready_split, 1);
+            if (rn) atomicAdd(&st->n_new_blocks, 1);
+        }
+    }
+    d.mark[y] = m;
+    ready_new[li] = rn;
+}
+
+// slopes of the cells about to split, on the pre-split tree (:169)
+template <int MODEL, int D>
+__global__ void k_split_slopes(Dev d, MConst c, const int32_t *__restrict__ list, int nl) {
+    const int li = blockIdx.x * blockDim.x + threadIdx.x;
+    if (li >= nl) return;
+    const int32_t y = list[li];
+    if (d.mark[y] & MK_READY) compute_slope_dev<MODEL, D>(d, c, y);
+}
+
+// Domain::split (domain.hpp:192-247) + child set-up (:174-195).  new_off is
+// the exclusive scan of "needs a new child block" over the list.
+template <int MODEL, int D>
+__global__ void k_split(Dev d, MConst c, const int32_t *__restrict__ list, int nl, const int32_t *new_off,
+                        int64_t n_before, int L) {
+    using S = Sz<MODEL, D>;
+    constexpr int NE = S::NE;
+    const int li = blockIdx.x * blockDim.x + threadIdx.x;
+    if (li >= nl) return;
+    const int64_t C = d.C;
+    const int32_t y = list[li];
+    if (!(d.mark[y] & MK_READY)) return;
+    int32_t base = d.child_base[y];
+    const bool reuse = base != 0;
+    if (!reuse) base = (int32_t) (n_before + (int64_t) S::NCH * new_off[li]);
+    double pc[NE], sl[D][NE], ctr[D];
+    for (int e = 0; e < NE; ++e) pc[e] = d.p[e * C + y];
+    for (int a = 0; a < D; ++a)
+        for (int e = 0; e < NE; ++e) sl[a][e] = d.slope[(a * NE + e) * C + y];
+    for (int a = 0; a < D; ++a) ctr[a] = d.center[a * C + y];
+    const double ps = d.size[y], pt = d.t[y];
+    const int plev = d.level[y];
+    for (int ch = 0; ch < S::NCH; ++ch) {
+        const int32_t x = base + ch;
+        double cc[D], p[NE], u[NE];
+        const double q = 0.25 * ps;
+        for (int a = 0; a < D; ++a) cc[a] = ctr[a] + (((ch >> a) & 1) ? 1.0 : -1.0) * q;
+        for (int e = 0; e < NE; ++e) p[e] = pc[e];
+        for (int a = 0; a < D; ++a) {
+            const double dx = cc[a] - ctr[a];
+            for (int e = 0; e < NE; ++e) p[e] = p[e] + sl[a][e] * dx;
+        }
+        cons_from_prim<MODEL, D>(c, p, u);
+        d.parent[x] = y;
+        if (!reuse) d.child_base[x] = 0;
+        for (int f = 0; f < 2 * D; ++f) {
+            d.nbr[f * C + x] = 0;
+            const int a = f >> 1, dir = f & 1;
+            d.bc[f * C + x] = (((ch >> a) & 1) == dir) ? d.bc[f * C + y] : 0;
+            for (int e = 0; e < S::NF; ++e) d.acc[(f * S::NF + e) * C + x] = 0.0;
+        }
+        d.level[x] = (int8_t) (plev + 1);
+        d.acti[x] = (int8_t) L;
+        d.cluster[x] = (int8_t) L;
+        d.buffer[x] = -1;
+        d.flags[x] = F_LEAF | F_ACTIVE | F_CAN_SPLIT | F_CAN_UNSPLIT;
+        d.mark[x] = 0;
+        for (int a = 0; a < D; ++a) d.center[a * C + x] = cc[a];
+        d.size[x] = 0.5 * ps;
+        d.t[x] = pt;
+        for (int e = 0; e < NE; ++e) {
+            d.p[e * C + x] = p[e];
+            d.u[e * C + x] = u[e];
+            d.rhs[e * C + x] = 0.0;
+        }
+        for (int a = 0; a < D; ++a)
+            for (int e = 0; e < NE; ++e) d.slope[(a * NE + e) * C + x] = sl[a][e];
+        for (int e = 0; e < S::NPX; ++e) d.k[e * C + x] = d.k[e * C + y];
+        for (int e = 0; e < S::NGX; ++e) d.t_slope[e * C + x] = d.t_slope[e * C + y];
+        for (int a = 0; a < D; ++a)
+            for (int e = 0; e < S::NGX; ++e) d.grad[(a * S::NGX + e) * C + x] = d.grad[(a * S::NGX + e) * C + y];
+        d.error[x] = -1.0;
+        d.ctime[x] = pt;
+        d.max_dt[x] = 0.0;
+        d.cfl[x] = 0.0;
+    }
+    d.child_base[y] = base;
+    d.flags[y] = (uint8_t) (d.flags[y] & ~(F_LEAF | F_SHOULD_SPLIT));
+}
+
+// Domain::unsplit (domain.hpp:296-325) of the parents whose children are
+// all marked (:198-220); one thread per parent (the one of child 0).
+template <int MODEL, int D>
+__device__ void interpolated_cell_acc(const Dev &d, int32_t idx, double &t, double *u, double *k, double (*g)[8],
+                                      double *ts, const MConst &c);
+
+template <int MODEL, int D>
+__global__ void k_unsplit(Dev d, MConst c, const int32_t *__restrict__ list, int nl, int L, Status *st) {
+    using S = Sz<MODEL, D>;
+    constexpr int NE = S::NE, NCH = S::NCH;
+    const int li = blockIdx.x * blockDim.x + threadIdx.x;
+    if (li >= nl) return;
+    const int64_t C = d.C;
+    const int32_t y = list[li];
+    if (!(d.mark[y] & MK_UNSPLIT_CHILD)) return;
+    const int32_t par = d.parent[y];
+    const int32_t cb = d.child_base[par];
+    if (cb != y) return;
+    for (int ch = 0; ch < NCH; ++ch) {
+        const int32_t x = cb + ch;
+        if (!(d.cluster[x] == L && (d.mark[x] & MK_UNSPLIT_CHILD))) return;
+    }
+    // interpolated_cell(par): domain.hpp:150-180 (children are reached
+    // through the recursion; gradient starts from the parent's stored one)
+    double t = 0.0, u[NE], k[S::NPX], g[D][8], ts[S::NGX];
+    for (int e = 0; e < NE; ++e) u[e] = 0.0;
+    for (int e = 0; e < S::NPX; ++e) k[e] = 0.0;
+    for (int e = 0; e < S::NGX; ++e) ts[e] = 0.0;
+    for (int a = 0; a < D; ++a)
+        for (int e = 0; e < S::NGX; ++e) g[a][e] = d.grad[(a * S::NGX + e) * C + par];
+    for (int ch = 0; ch < NCH; ++ch) interpolated_cell_acc<MODEL, D>(d, cb + ch, t, u, k, g, ts, c);
+    t /= NCH;
+    for (int e = 0; e < S::NGX; ++e) ts[e] = ts[e] / NCH;
+    for (int e = 0; e < NE; ++e) u[e] = u[e] / NCH;
+    double p[NE];
+    prim_from_cons<MODEL, D>(c, u, p);
+    for (int e = 0; e < S::NPX; ++e) k[e] = k[e] / NCH;
+    for (int a = 0; a < D; ++a)
+        for (int e = 0; e < S::NGX; ++e) g[a][e] = g[a][e] / NCH;
+    double err = 0.0;
+    for (int ch = 0; ch < NCH; ++ch) {
+        const int32_t x = cb + ch;
+        err += d.error[x];
+        d.flags[x] = (uint8_t) (d.flags[x] | F_GARBAGE | F_LEAF);
+    }
+    for (int f = 0; f < 2 * D; ++f) {
+        d.nbr[f * C + par] = 0;
+        for (int e = 0; e < S::NF; ++e) d.acc[(f * S::NF + e) * C + par] = 0.0;
+    }
+    d.t[par] = t;
+    for (int e = 0; e < NE; ++e) {
+        d.u[e * C + par] = u[e];
+        d.p[e * C + par] = p[e];
+    }
+    for (int e = 0; e < S::NPX; ++e) d.k[e * C + par] = k[e];
+    for (int e = 0; e < S::NGX; ++e) d.t_slope[e * C + par] = ts[e];
+    for (int a = 0; a < D; ++a)
+        for (int e = 0; e < S::NGX; ++e) d.grad[(a * S::NGX + e) * C + par] = g[a][e];
+    d.error[par] = err;
+    d.ctime[par] = t;
+    d.flags[par] = (uint8_t) (d.flags[par] | F_LEAF | F_ACTIVE | F_CAN_SPLIT);
+    d.acti[par] = (int8_t) L;
+    d.cluster[par] = (int8_t) L;
+    atomicAdd(&st->n_unsplit, 1);
+}
+
+// Accumulates interpolated_cell(idx) of one child into the running sums
+// (domain.hpp:162-169); non-leaf children recurse.
+template <int MODEL, int D>
+__device__ void interpolated_cell_acc(const Dev &d, int32_t idx, double &t, double *u, double *k, double (*g)[8],
+                                      double *ts, const MConst &c) {
+    using S = Sz<MODEL, D>;
+    constexpr int NE = S::NE;
+    const int64_t C = d.C;
+    double ct, cu[NE], ck[S::NPX], cg[D][8], cts[S::NGX];
+    if (d.flags[idx] & F_LEAF) {
+        ct = d.t[idx];
+        for (int e = 0; e < NE; ++e) cu[e] = d.u[e * C + idx];
+        for (int e = 0; e < S::NPX; ++e) ck[e] = d.k[e * C + idx];
+        for (int a = 0; a < D; ++a)
+            for (int e = 0; e < S::NGX; ++e) cg[a][e] = d.grad[(a * S::NGX + e) * C + idx];
+        for (int e = 0; e < S::NGX; ++e) cts[e] = d.t_slope[e * C + idx];
+    } else {
+        ct = 0.0;
+        for (int e = 0; e < NE; ++e) cu[e] = 0.0;
+        for (int e = 0; e < S::NPX; ++e) ck[e] = 0.0;
+        for (int e = 0; e < S::NGX; ++e) cts[e] = 0.0;
+        for (int a = 0; a < D; ++a)
+            for (int e = 0; e < S::NGX; ++e) cg[a][e] = d.grad[(a * S::NGX + e) * C + idx];
+        const int32_t cb = d.child_base[idx];
+        for (int ch = 0; ch < S::NCH; ++ch) interpolated_cell_acc<MODEL, D>(d, cb + ch, ct, cu, ck, cg, cts, c);
+        ct /= S::NCH;
+        for (int e = 0; e < S::NGX; ++e) cts[e] = cts[e] / S::NCH;
+        for (int e = 0; e < NE; ++e) cu[e] = cu[e] / S::NCH;
+        for (int e = 0; e < S::NPX; ++e) ck[e] = ck[e] / S::NCH;
+        for (int a = 0; a < D; ++a)
+            for (int e = 0; e < S::NGX; ++e) cg[a][e] = cg[a][e] / S::NCH;
+    }
+    t += ct;
+    for (int e = 0; e < NE; ++e) u[e] = u[e] + cu[e];
+    for (int e = 0; e < S::NPX; ++e) k[e] = k[e] + ck[e];
+    for (int a = 0; a < D; ++a)
+        for (int e = 0; e < S::NGX; ++e) g[a][e] = g[a][e] + cg[a][e];
+    for (int e = 0; e < S::NGX; ++e) ts[e] = ts[e] + cts[e];
+}
+
+__global__ void k_clear_marks(Dev d, const int32_t *__restrict__ list, int nl) {
+    const int li = blockIdx.x * blockDim.x + threadIdx.x;
+    if (li >= nl) return;
+    d.mark[list[li]] = 0;
+}
+
+// update_neighbors (domain.hpp:249-293) for every active leaf
+template <int D>
+__global__ void __launch_bounds__(256) k_neighbors(Dev d, MConst c, int64_t n) {
+    const int64_t i = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < 1 || i >= n) return;
+    const uint8_t fl = d.flags[i];
+    if (!(fl & F_LEAF) || !(fl & F_ACTIVE) || (fl & F_GARBAGE)) return;
+    const int64_t C = d.C;
+    const int lev = d.level[i];
+    const double s = d.size[i];
+    for (int dim = 0; dim < D; ++dim)
+        for (int dir = 0; dir < 2; ++dir) {
+            const int f = 2 * dim + dir;
+            if (d.bc[f * C + i]) continue;
+            double loc[D];
+            for (int a = 0; a < D; ++a) loc[a] = d.center[a * C + i];
+            loc[dim] = loc[dim] + s * (2 * dir - 1.0);
+            const int32_t nb = dom_find<D>(d, c, loc, lev);
+            if (!nb || !(d.flags[nb] & F_ACTIVE)) continue;
+            d.nbr[f * C + i] = nb;
+            if (d.level[nb] < lev - 1) atomic_or_u8(&d.flags[nb], F_SHOULD_SPLIT);
+        }
+}
+
+// ------------------------------------------------------- LTS reassignment
+// impl/adaptive_refinement.hpp:264-322
+template <int MODEL, int D>
+__global__ void __launch_bounds__(256) k_reassign(Dev d, MConst c, const int32_t *__restrict__ list, int nl, int L,
+                                                   double dt, int counter, Status *st) {
+    using S = Sz<MODEL, D>;
+    const int li = blockIdx.x * blockDim.x + threadIdx.x;
+    if (li >= nl) return;
+    const int64_t C = d.C;
+    const int32_t y = list[li];
+    double p[S::NE];
+    for (int e = 0; e < S::NE; ++e) p[e] = d.p[e * C + y];
+    const double mdt = max_delta_time<MODEL, D>(c, p, d.size[y]);
+    double cfl = dt / mdt;
+    d.max_dt[y] = mdt;
+    const uint8_t fl = d.flags[y];
+    if (!(fl & F_LEAF) || (fl & F_GARBAGE)) {
+        d.cfl[y] = cfl;
+        d.cluster[y] = -1;
+        atomicAdd(&st->dropped, 1);
+        return;
+    }
+    if (cfl < 0.5 * c.max_cfl && counter == 1 && L > 0) {
+        d.cluster[y] = -1;
+        d.buffer[y] = (int8_t) (L - 1);
+        cfl *= 2;
+        atomicAdd(&st->lowered, 1);
+    } else if (cfl > c.max_cfl && L < 15) {
+        const int raise = (int) (log2(cfl / c.max_cfl) + 1);
+        if (L + raise < 16) {
+            d.acti[y] = (int8_t) (L + raise);
+            cfl /= 1 << raise;
+            d.cluster[y] = (int8_t) (L + raise);
+            atomicAdd(&st->raised[L + raise], 1);
+        } else {
+            d.cluster[y] = -1;
+            atomicAdd(&st->dropped, 1);
+        }
+    } else {
+        atomicAdd(&st->kept, 1);
+    }
+    d.cfl[y] = cfl;
+}
+
+// flush_acti_buffer: impl/acti_utils.hpp:43-50
+__global__ void k_flush(Dev d, int64_t n, int L) {
+    const int64_t i = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < 1 || i >= n) return;
+    if (d.buffer[i] == L) {
+        d.buffer[i] = -1;
+        d.cluster[i] = (int8_t) L;
+        d.acti[i] = (int8_t) L;
+    }
+}
+
+// ------------------------------------------------------------------ init
+// Integrator constructor, first loop: integrator/integrator.hpp:100-122
+template <int MODEL, int D>
+__global__ void k_init_cells(Dev d, MConst c, int64_t n) {
+    using S = Sz<MODEL, D>;
+    const int64_t i = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= d.C) return;
+    d.scratch[i] = -INFINITY;
+    if (i < 1 || i >= n) return;
+    const uint8_t fl = d.flags[i];
+    if (!(fl & F_LEAF) || !(fl & F_ACTIVE)) return;
+    const int64_t C = d.C;
+    double p[S::NE], u[S::NE];
+    for (int e = 0; e < S::NE; ++e) p[e] = d.p[e * C + i];
+    cons_from_prim<MODEL, D>(c, p, u);
+    for (int e = 0; e < S::NE; ++e) d.u[e * C + i] = u[e];
+    if constexpr (MODEL == MULTI) {
+        d.k[0 * C + i] = c.mu1 * p[S::IA] + c.mu2 * (1 - p[S::IA]);
+        d.k[S::IALPHAS * C + i] = p[S::IA];
+    }
+    for (int e = 0; e < S::NGX; ++e) d.t_slope[e * C + i] = 0.0;
+    const double m = max_delta_time<MODEL, D>(c, p, d.size[i]);
+    d.max_dt[i] = m;
+    d.scratch[i] = m;
+}
+
+// assign_to_acti_clusters: impl/acti_utils.hpp:10-24
+__global__ void k_assign(Dev d, int64_t n, double global_dt, double max_cfl) {
+    const int64_t i = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < 1 || i >= n) return;
+    const uint8_t fl = d.flags[i];
+    if (!(fl & F_ACTIVE) || !(fl & F_LEAF) || (fl & F_GARBAGE)) return;
+    double cfl = global_dt / d.max_dt[i];
+    int lv = (int) log2(cfl / max_cfl);
+    lv = lv < 0 ? 0 : (15 < lv ? 15 : lv);
+    d.acti[i] = (int8_t) lv;
+    d.cfl[i] = cfl / (1 << lv);
+    d.cluster[i] = (int8_t) lv;
+}
+
+__global__ void k_level_hist(const int8_t *__restrict__ cluster, int64_t n, unsigned long long *hist) {
+    __shared__ unsigned int h[32];
+    if (threadIdx.x < 32) h[threadIdx.x] = 0;
+    __syncthreads();
+    for (int64_t i = (int64_t) blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t) gridDim.x * blockDim.x) {
+        const int cl = cluster[i];
+        if (i >= 1 && cl >= 0 && cl < 16) atomicAdd(&h[cl], 1u);
+    }
+    __syncthreads();
+    if (threadIdx.x < 16 && h[threadIdx.x]) atomicAdd(&hist[threadIdx.x], (unsigned long long) h[threadIdx.x]);
+}
+
+}  // namespace dacti

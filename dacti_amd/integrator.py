@@ -1,0 +1,120 @@
+"""Integrator facade over the C ABI (include/dacti_gpu.h).
+
+Mirrors dacti::integrator::Integrator<Model, Limiter, ErrorEstimator>
+(src/lib/integrator/integrator.hpp:14-225): construction from a case and its
+initial tree runs the constructor's initialisation on the GPU, integrate()
+is one global step (impl/integrate.hpp:11-87) and the read-side calls
+(n_active_cells, get_nodes, for_each_active_cell, global_dt,
+max_acti_level) match the reference's names and meaning.  Errors raised by
+the C ABI become RuntimeError; a NaN state is reported through
+force_break, as the reference does (impl/cell_update.hpp:52-57).
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from .scene import build_tree
+
+METADATA_NAMES = ("acti_level", "error", "max_dt", "cfl")  # integrator.hpp:30
+MAX_LEVEL = 16
+
+
+class Integrator:
+    NumMeta = 4
+    MAX_LEVEL = MAX_LEVEL
+    METADATA_NAMES = METADATA_NAMES
+
+    def __init__(self, case, tree=None):
+        self.case = case
+        self.lib = _lib.load()
+        self.tree = tree if tree is not None else build_tree(case)
+        self._cs = _lib.case_struct(case)
+        self._ts = _lib.tree_struct(self.tree)
+        h = C.c_void_p()
+        rc = self.lib.dacti_create(C.byref(self._cs), C.byref(self._ts), C.byref(h))
+        if rc != 0:
+            raise RuntimeError(f"dacti_create failed ({rc}): {self.lib.dacti_last_error(None).decode()}")
+        self.h = h
+        self.NumDims = case.dim
+
+    def _check(self, rc, what):
+        if rc != 0:
+            raise RuntimeError(f"{what} failed ({rc}): {self.lib.dacti_last_error(self.h).decode()}")
+
+    def integrate(self, substep_hook=None, signal=True):
+        """One global step; returns (global_dt, force_break)."""
+        dt = C.c_double()
+        fb = C.c_int32()
+        self._check(self.lib.dacti_integrate(self.h, int(bool(signal)), C.byref(dt), C.byref(fb)), "integrate")
+        if substep_hook is not None:
+            s = self.stats()
+            substep_hook(self, s.max_acti_level, None, s.n_substeps, dt.value)
+        return dt.value, bool(fb.value)
+
+    def run(self, steps, signal=True):
+        done = C.c_int64()
+        self._check(self.lib.dacti_run(self.h, steps, int(bool(signal)), C.byref(done)), "run")
+        return done.value
+
+    def n_active_cells(self):
+        n = C.c_int64()
+        self._check(self.lib.dacti_n_active(self.h, C.byref(n)), "n_active")
+        return n.value
+
+    def n_nodes(self):
+        n = C.c_int64()
+        self._check(self.lib.dacti_n_nodes(self.h, C.byref(n)), "n_nodes")
+        return n.value
+
+    def get_nodes(self):
+        """Node pool as host arrays (node-major), Integrator::get_nodes()."""
+        n = self.n_nodes()
+        arrs = _lib.alloc_node_arrays(n, self.case.model, self.case.dim)
+        v = _lib.node_view(arrs)
+        self._check(self.lib.dacti_export_nodes(self.h, C.byref(v)), "export_nodes")
+        return arrs
+
+    def for_each_active_cell(self, action):
+        """action(i, node_index, nodes) over the leaves of the LTS clusters and
+        buffers, in level order (integrator.hpp:172-191)."""
+        nodes = self.get_nodes()
+        cl = nodes["cluster"]
+        i = 0
+        for key in list(range(MAX_LEVEL)) + [MAX_LEVEL + l for l in range(MAX_LEVEL)]:
+            for idx in np.nonzero(cl == key)[0]:
+                action(i, int(idx), nodes)
+                i += 1
+
+    def stats(self):
+        s = _lib.StatsStruct()
+        self._check(self.lib.dacti_get_stats(self.h, C.byref(s)), "get_stats")
+        return s
+
+    def set_timing(self, on):
+        self._check(self.lib.dacti_set_timing(self.h, int(bool(on))), "set_timing")
+
+    @property
+    def global_dt(self):
+        return self.stats().global_dt
+
+    @property
+    def max_acti_level(self):
+        return self.stats().max_acti_level
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.dacti_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()

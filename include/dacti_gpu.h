@@ -1,0 +1,188 @@
+/*
+ * dacti_gpu.h -- C ABI of the MI355X octree leaf-update engine.
+ *
+ * The reference solver exposes this path as a header-only C++ template,
+ * dacti::integrator::Integrator<Model, Limiter, ErrorEstimator>
+ * (src/lib/integrator/integrator.hpp:14-225), instantiated by each example
+ * main (src/examples/multi2d.cpp:19-23).  It has no FFI; the entry points
+ * below are the ones a C-ABI binding of that class needs, one per public
+ * member it exercises.  All pointers are plain host pointers; every call
+ * returns 0 on success or a negative DACTI_E_* code, with the message
+ * available from dacti_last_error().  A handle owns its device buffers and
+ * its HIP stream; handles share no state and one host thread drives each.
+ */
+#ifndef DACTI_GPU_H
+#define DACTI_GPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct dacti_handle dacti_handle;
+
+/* status codes */
+#define DACTI_OK 0
+#define DACTI_E_ARG (-1)     /* invalid argument */
+#define DACTI_E_HIP (-2)     /* HIP runtime error */
+#define DACTI_E_MEMORY (-3)  /* device allocation failed */
+#define DACTI_E_STATE (-4)   /* call not valid in the handle's state */
+
+/* model ids: model/euler.hpp, model/navier_stokes.hpp, model/multi.hpp */
+#define DACTI_MODEL_EULER 0
+#define DACTI_MODEL_NAVIER_STOKES 1
+#define DACTI_MODEL_MULTI 2
+
+/* limiter ids: integrator/slope_limiters.hpp:7-57 */
+#define DACTI_LIMITER_MINMOD 0
+#define DACTI_LIMITER_UNLIMITED 1
+#define DACTI_LIMITER_VAN_LEER 2
+#define DACTI_LIMITER_SUPERBEE 3
+#define DACTI_LIMITER_VON_ALBA_2 4
+#define DACTI_LIMITER_KOREN 5
+#define DACTI_LIMITER_FIRST_ORDER 6
+
+/* boundary types: integrator/boundary.hpp:21-30 */
+#define DACTI_BC_ZERO_GRADIENT 0
+#define DACTI_BC_FREESLIP 1
+#define DACTI_BC_NOSLIP 2
+#define DACTI_BC_INFLOW 3
+#define DACTI_BC_INFLOW_R 4
+#define DACTI_BC_OUTFLOW 5
+#define DACTI_BC_DIRICHLET 6
+
+/* node flag bits (tree exchange format) */
+#define DACTI_F_LEAF 1
+#define DACTI_F_ACTIVE 2
+#define DACTI_F_GARBAGE 4
+#define DACTI_F_CAN_SPLIT 8
+#define DACTI_F_CAN_UNSPLIT 16
+#define DACTI_F_SHOULD_SPLIT 32
+#define DACTI_F_BOUNDARY 64
+
+#define DACTI_MAX_ZONES 8
+#define DACTI_MAX_LEVEL 16
+
+/*
+ * Case parameters.  These are the Config<M> fields the integrator reads
+ * (integrator/integrator.hpp:80-86, impl/flux.hpp:68-76,
+ * scene/scene.hpp:50-61) and the model constants that the reference keeps
+ * as process-global statics (model/multi.hpp:50-62,
+ * model/navier_stokes.hpp:27-31); here they belong to one handle.
+ */
+typedef struct dacti_case {
+    int32_t model, dim, limiter, max_refinement_level;
+    double side_length;
+    double active_size[3];
+    int32_t periodic[3];
+    int32_t n_zones;
+    double max_cfl, max_dt, min_error, max_error;
+    double weights[8];
+    /* euler / navier_stokes: gamma (1.4), mu, lambda, cp (1005), R (287) */
+    double gamma, ns_mu, ns_lambda, ns_cp, ns_R;
+    /* multi: sigma, stiffened-gas pair, heat capacities, viscosities */
+    double sigma, gamma1, gamma2, P1, P2, cp1, cp2, cv1, cv2, mu1, mu2, lambda1, lambda2;
+    /* domain boundary conditions, face index 2*dim+dir */
+    int32_t bc_type[6];
+    double bc_p[6][8];
+    double bc_fade_in[6];
+    /* refinement zones (axis-aligned boxes) */
+    double zone_x0[DACTI_MAX_ZONES][3];
+    double zone_x1[DACTI_MAX_ZONES][3];
+    int32_t zone_level[DACTI_MAX_ZONES];
+} dacti_case;
+
+/*
+ * Initial tree: the node pool after Scene::init_domain
+ * (scene/scene.cpp:369-431), in the layout of integrator/domain.hpp:14-63.
+ * Node 0 is the null node, node 1 the root.  A node's 2^dim children occupy
+ * indices child_base .. child_base + 2^dim - 1 (0 = no children).
+ * bc[i*2*dim + f] is 0 for no boundary or 1 + the domain face index of the
+ * boundary condition that face carries; p holds the primitive initial state
+ * of active leaves (num_eqns values per node).
+ */
+typedef struct dacti_tree {
+    int64_t n_nodes;
+    const int32_t *parent;
+    const int32_t *child_base;
+    const int8_t *tree_level;
+    const uint8_t *flags;
+    const uint8_t *bc;
+    const double *center; /* n_nodes * dim */
+    const double *size;
+    const double *p;      /* n_nodes * num_eqns */
+} dacti_tree;
+
+/*
+ * Node-pool view filled by dacti_export_nodes (caller-allocated, n_nodes
+ * entries; sizes per node: nbr 2*dim, center dim, p/u/rhs num_eqns,
+ * slope dim*num_eqns, k max(num_parm,1), grad dim*max(num_grad,1), meta 4
+ * = {acti_level, error, max_dt, cfl}, integrator/integrator.hpp:30).
+ * cluster: LTS level whose cluster holds the node, 16 + level for a node
+ * waiting in an acti buffer, -1 for none.  Any pointer may be NULL.
+ */
+typedef struct dacti_node_view {
+    int32_t *parent, *child_base, *nbr;
+    int8_t *tree_level, *acti_level, *cluster;
+    uint8_t *flags;
+    double *center, *size, *t, *p, *u, *slope, *rhs, *k, *grad, *meta;
+} dacti_node_view;
+
+/* counters and timers of a handle */
+typedef struct dacti_stats {
+    int64_t n_nodes, n_active;
+    int64_t n_substeps, n_global_steps;
+    int64_t cell_updates;       /* leaf-cell updates: sum of cluster sizes per substep */
+    int64_t flux_launches;      /* launches of the fused flux+update kernel */
+    double flux_kernel_ms;      /* HIP-event time of those launches (when timing is on) */
+    double substep_kernel_ms;   /* HIP-event time of all per-substep kernels (when timing is on) */
+    double global_dt;
+    int32_t max_acti_level;
+    int32_t pad;
+} dacti_stats;
+
+/* sizes of a model: out[0..4] = dim, num_eqns, num_flux, num_parm, num_grad */
+int dacti_model_sizes(int32_t model, int32_t dim, int32_t *out);
+
+/*
+ * Integrator constructor: integrator/integrator.hpp:72-163.  Uploads the
+ * tree and, on the GPU, sets conserved state and parameters, the CFL limit
+ * of each leaf and its max-reduction to global_dt, neighbours, slopes, the
+ * multiphase interface fields, and the initial LTS clusters.
+ */
+int dacti_create(const dacti_case *c, const dacti_tree *t, dacti_handle **out);
+
+/* Integrator::integrate(hook, signal, force_break): impl/integrate.hpp:11-87.
+ * One global step; *global_dt receives the (constant) global step and
+ * *force_break is set to 1 when a NaN pressure or density appeared. */
+int dacti_integrate(dacti_handle *h, int32_t signal, double *global_dt, int32_t *force_break);
+
+/* `steps` calls of dacti_integrate; *completed = steps done before a break. */
+int dacti_run(dacti_handle *h, int64_t steps, int32_t signal, int64_t *completed);
+
+/* Integrator::n_active_cells(): integrator/integrator.hpp:193-202 */
+int dacti_n_active(dacti_handle *h, int64_t *n);
+
+/* size of the node pool (Integrator::get_nodes().size(), :165) */
+int dacti_n_nodes(dacti_handle *h, int64_t *n);
+
+/* Integrator::get_nodes() / for_each_active_cell(): :165-191 */
+int dacti_export_nodes(dacti_handle *h, const dacti_node_view *v);
+
+int dacti_get_stats(dacti_handle *h, dacti_stats *s);
+
+/* turn HIP-event timing of the per-substep kernels on or off */
+int dacti_set_timing(dacti_handle *h, int32_t on);
+
+int dacti_destroy(dacti_handle *h);
+
+/* message of the last failed call on h (or of the last failed create when
+ * h is NULL); never NULL */
+const char *dacti_last_error(dacti_handle *h);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
