@@ -1,0 +1,118 @@
+#!/usr/bin/env python3
+"""Benchmark: leaf-cell updates/s of the FP64 octree leaf update on one MI355X.
+
+Workload: 3D single-phase Euler in a periodic unit box on a uniform octree at
+level 8 (256^3 = 16,777,216 leaves), smoothed random density and a
+divergence-free velocity field of RMS Mach 0.5, p = 1, gamma = 1.4, LTS on,
+no refinement (HLLC-type flux + MUSCL-Hancock, minmod).  A step is one
+global step of integrate(); a leaf-cell update is one leaf advanced by one
+LTS substep.  Inputs are resident in HBM before the timed region.
+
+Prints one JSON line (see the keys below).  `--gpus N` with N > 1 exits: this
+engine runs on one GPU.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+B_ALG = 8 * (2 * 5 + 2)  # bytes per leaf-cell update: state in + out, cell time in + out (euler<3>: NE = 5)
+HBM_PEAK_GBS = 8000.0
+
+
+def cpu_baseline(case, tree, seconds_target=20.0):
+    """The oracle (C restatement, OpenMP) on the same initial tree: one global step."""
+    from oracle.binding import Oracle
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    o = Oracle(case, tree)
+    u0 = o.cell_updates
+    t0 = time.perf_counter()
+    o.integrate()
+    el = time.perf_counter() - t0
+    n = o.cell_updates - u0
+    o.close()
+    return {"value": n / el, "unit": "cell-updates/s", "cores": threads, "kind": "port",
+            "sample": f"one global step of the same 256^3 workload from the same initial state "
+                      f"({n} leaf-cell updates in {el:.1f} s, OpenMP {threads} threads)"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--level", type=int, default=8)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this engine runs on one GPU; --gpus {args.gpus} is not supported", file=sys.stderr)
+        sys.exit(2)
+
+    from dacti_amd import scene, workloads
+    from dacti_amd.integrator import Integrator
+
+    t_build = time.perf_counter()
+    case = workloads.periodic_box(level=args.level)
+    tree = scene.build_tree(case)
+    print(f"[bench] tree built: {tree.n} nodes in {time.perf_counter() - t_build:.1f} s", file=sys.stderr, flush=True)
+    g = Integrator(case, tree)
+    print("[bench] integrator created", file=sys.stderr, flush=True)
+    for w in range(args.warmup):
+        g.integrate()
+        print(f"[bench] warmup step {w} done", file=sys.stderr, flush=True)
+    g.set_timing(True)
+    s0 = g.stats()
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        _, fb = g.integrate()
+        if fb:
+            raise RuntimeError("NaN state (force_break)")
+    el = time.perf_counter() - t0
+    s1 = g.stats()
+    print(f"[bench] {args.steps} timed steps in {el:.3f} s", file=sys.stderr, flush=True)
+    updates = s1.cell_updates - s0.cell_updates
+    value = updates / el
+    launches = s1.flux_launches - s0.flux_launches
+    kern_ms = s1.substep_kernel_ms - s0.substep_kernel_ms
+    # one "launch" of the dominant work = all kernels of one level substep
+    alg_bytes_per_launch = B_ALG * updates / max(launches, 1)
+    avg_ms = kern_ms / max(launches, 1)
+    achieved = alg_bytes_per_launch / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
+    g.close()
+    base = None
+    if not args.no_cpu_baseline:
+        print("[bench] timing the CPU baseline", file=sys.stderr, flush=True)
+        base = cpu_baseline(case, tree)
+    out = {
+        "metric": "cell-updates/s (HLLC+MUSCL FP64) & %HBM roofline; learned/batched vs plain",
+        "value": value,
+        "unit": "cell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": el * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"3D Euler periodic box, uniform octree level {args.level} "
+                               f"({(1 << args.level) ** 3} leaves), LTS on, no refinement",
+                   "leaves": (1 << args.level) ** 3, "model": "euler<3>", "limiter": "minmod",
+                   "max_cfl": case.max_cfl, "global_dt": s1.global_dt, "max_acti_level": s1.max_acti_level,
+                   "cell_updates_timed": updates, "substeps_timed": launches},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "one level substep (flux, update, slope, flag, reassign kernels)",
+                     "bytes_per_update": B_ALG, "avg_launch_ms": avg_ms},
+        "cpu_baseline": base,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -361,8 +361,9 @@ __global__ void __launch_bounds__(256) k_flux(Dev d, MConst c, const int32_t *__
                 const int nlev = d.level[nb];
                 const int nacti = d.acti[nb];
                 const int ncl = d.cluster[nb];
-                if (nlev == ylev && nacti > L) {
-                    // deposits of a same-level neighbour on a finer time level
+                if (nacti > L) {
+                    // deposits of a neighbour on a finer time level (or of the
+                    // cells it was split from or coarsened from)
 #pragma unroll
                     for (int e = 0; e < NF; ++e) {
                         cc[e] = d.acc[(f * NF + e) * C + y];

@@ -36,6 +36,7 @@
 #include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
+#include <stdio.h>
 
 #include "oracle_common.h"
 
@@ -1009,11 +1010,13 @@ static void flux_phase(o_state *S, const int32_t *list, int64_t nl, int L) {
                 for (int i = 0; i < NF; ++i) ny->acc[f][i] = 0.0;
                 int32_t nb = ny->nbr[f];
                 {
-                    /* Received flux can only come from a face neighbour on a
-                     * finer time level; count any that appears elsewhere. */
+                    /* Received flux can only come from a face neighbour that
+                     * is now on a finer time level (possibly the parent or
+                     * children of the cells that deposited it); count any
+                     * that appears elsewhere. */
                     int may = 0;
                     if (nb != 0 && S->tree[nb].leaf) {
-                        may = S->tree[nb].tree_level == ny->tree_level && S->tree[nb].acti > L;
+                        may = S->tree[nb].acti > L;
                     } else if (nb != 0) {
                         int32_t subs[O_MAX_SUB];
                         int ns = sub_face_leaves(S, nb, dim, 1 - dir, subs, 0);
@@ -1026,6 +1029,13 @@ static void flux_phase(o_state *S, const int32_t *list, int64_t nl, int L) {
                     if (nz && !may) {
 #pragma omp atomic
                         S->acc_violations++;
+                        if (getenv("ORACLE_DEBUG"))
+                            fprintf(stderr,
+                                    "acc violation: L=%d y=%d lev=%d acti=%d face=%d nb=%d nbleaf=%d nblev=%d "
+                                    "nbacti=%d nbcl=%d nbbuf=%d\n",
+                                    L, y, ny->tree_level, ny->acti, f, nb, nb ? S->tree[nb].leaf : -1,
+                                    nb ? S->tree[nb].tree_level : -1, nb ? S->tree[nb].acti : -1,
+                                    nb ? S->tree[nb].cluster : -1, nb ? S->tree[nb].buffer : -1);
                     }
                 }
                 if (nb == 0) {
@@ -1572,12 +1582,16 @@ void *OPFX(create)(const o_params *P, int64_t n, const int32_t *parent, const in
             gmax = smax(gmax, nd->max_dt);
         }
     }
+    /* neighbours and slopes of every leaf: independent per leaf */
+#pragma omp parallel for schedule(guided, 64)
     for (int64_t i = 1; i < n; ++i) {
         o_node *nd = &S->tree[i];
-        if (nd->leaf && nd->active) {
-            update_neighbors_leaf(S, (int32_t) i);
-            compute_slope(S, (int32_t) i);
-        }
+        if (nd->leaf && nd->active) update_neighbors_leaf(S, (int32_t) i);
+    }
+#pragma omp parallel for schedule(guided, 64)
+    for (int64_t i = 1; i < n; ++i) {
+        o_node *nd = &S->tree[i];
+        if (nd->leaf && nd->active) compute_slope(S, (int32_t) i);
     }
 #if OMODEL == O_MULTI
     {
