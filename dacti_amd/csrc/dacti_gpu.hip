@@ -70,6 +70,7 @@ struct dacti_handle {
     bool timing = false;
     hipStream_t stream = nullptr;
     int32_t *d_list = nullptr;
+    int32_t *d_list2 = nullptr;
     int32_t *d_off = nullptr;
     uint8_t *d_rn = nullptr;
     int32_t *d_nsel = nullptr;
@@ -103,6 +104,7 @@ std::vector<Field> fields_of(dacti_handle *h, Dev &d) {
         {(void **) &d.acti, 1, 1},           {(void **) &d.cluster, 1, 1},
         {(void **) &d.buffer, 1, 1},         {(void **) &d.flags, 1, 1},
         {(void **) &d.bc, 1, s.NFACE},       {(void **) &d.mark, 1, 1},
+        {(void **) &d.nextc, 1, 1},
         {(void **) &d.center, 8, s.D},       {(void **) &d.size, 8, 1},
         {(void **) &d.t, 8, 1},              {(void **) &d.p, 8, s.NE},
         {(void **) &d.u, 8, s.NE},           {(void **) &d.rhs, 8, s.NE},
@@ -156,6 +158,8 @@ int alloc_pool(dacti_handle *h, int64_t C, Dev *old, int64_t n) {
     }
     // scratch buffers sized by capacity
     if (h->d_list) (void) hipFree(h->d_list);
+    if (h->d_list2) (void) hipFree(h->d_list2);
+    h->d_list2 = nullptr;
     if (h->d_off) (void) hipFree(h->d_off);
     if (h->d_rn) (void) hipFree(h->d_rn);
     if (h->d_tmp) (void) hipFree(h->d_tmp);
@@ -164,6 +168,7 @@ int alloc_pool(dacti_handle *h, int64_t C, Dev *old, int64_t n) {
     h->d_rn = nullptr;
     h->d_tmp = nullptr;
     HIPCHK(h, hipMalloc(&h->d_list, sizeof(int32_t) * (size_t) C));
+    HIPCHK(h, hipMalloc(&h->d_list2, sizeof(int32_t) * (size_t) C));
     HIPCHK(h, hipMalloc(&h->d_off, sizeof(int32_t) * (size_t) C));
     HIPCHK(h, hipMalloc(&h->d_rn, (size_t) C));
     size_t b1 = 0, b2 = 0, b3 = 0;
@@ -176,6 +181,7 @@ int alloc_pool(dacti_handle *h, int64_t C, Dev *old, int64_t n) {
     HIPCHK(h, hipcub::DeviceReduce::Max(nullptr, b3, nd.scratch, (double *) nd.tot, (int) C, h->stream));
     h->tmp_bytes = std::max(b1, std::max(b2, b3)) + 1024;
     HIPCHK(h, hipMalloc(&h->d_tmp, h->tmp_bytes));
+    nd.err = &h->d_status->dev_error;
     h->d = nd;
     return DACTI_OK;
 }
@@ -197,10 +203,10 @@ struct ClusterIs {
 };
 
 // sorted list of the node ids whose cluster is L (acti_clusters[L])
-int build_list(dacti_handle *h, int L, int64_t *nl) {
+int build_list(dacti_handle *h, int L, int64_t *nl, int32_t *out = nullptr) {
     hipcub::CountingInputIterator<int32_t> it(1);
     size_t bytes = h->tmp_bytes;
-    HIPCHK(h, hipcub::DeviceSelect::If(h->d_tmp, bytes, it, h->d_list, h->d_nsel, (int) (h->n - 1),
+    HIPCHK(h, hipcub::DeviceSelect::If(h->d_tmp, bytes, it, out ? out : h->d_list, h->d_nsel, (int) (h->n - 1),
                                        ClusterIs{h->d.cluster, L}, h->stream));
     HIPCHK(h, hipMemcpyAsync(h->h_nsel, h->d_nsel, sizeof(int32_t), hipMemcpyDeviceToHost, h->stream));
     HIPCHK(h, hipStreamSynchronize(h->stream));
@@ -307,7 +313,10 @@ struct Ops {
         HIPCHK(h, hipMemsetAsync(h->d_status, 0, sizeof(Status), h->stream));
         if (nl > 0) {
             if (h->timing) HIPCHK(h, hipEventRecord(h->ev[0], h->stream));
-            k_flux<MODEL, D><<<blocks(nl), 256, 0, h->stream>>>(h->d, h->mc, h->d_list, nl, L, dt);
+            k_flux_simple<MODEL, D><<<blocks(nl), 256, 0, h->stream>>>(h->d, h->mc, h->d_list, nl, L, dt, h->d_list2,
+                                                                       &h->d_status->n_rest);
+            k_flux<MODEL, D><<<std::min<unsigned>(blocks(nl), 1024), 256, 0, h->stream>>>(h->d, h->mc, h->d_list2,
+                                                                                      &h->d_status->n_rest, L, dt);
             k_update<MODEL, D><<<blocks(nl), 256, 0, h->stream>>>(h->d, h->mc, h->d_list, nl, dt, signal, h->d_status);
             HIPCHK(h, hipGetLastError());
             if (h->timing) HIPCHK(h, hipEventRecord(h->ev[1], h->stream));
@@ -319,29 +328,36 @@ struct Ops {
                 *force_break = 1;
                 return 1;
             }
-            k_slope<MODEL, D><<<blocks(nl), 256, 0, h->stream>>>(h->d, h->mc, h->d_list, nl);
+            // slopes, refinement marks and LTS decisions in one sweep
+            k_sweep<MODEL, D><<<blocks(nl), 256, 0, h->stream>>>(h->d, h->mc, h->d_list, nl, L, dt,
+                                                                 h->acti_counter[L], 0, h->d_status);
             HIPCHK(h, hipGetLastError());
             rc = chain(h, nl);
             if (rc) return rc;
-            // adaptive_refinement(L): impl/adaptive_refinement.hpp:6-323
-            k_flag<MODEL, D><<<blocks(nl), 256, 0, h->stream>>>(h->d, h->mc, h->d_list, nl, h->acti_counter[L],
-                                                                h->d_status);
-            HIPCHK(h, hipGetLastError());
             rc = read_status(h);
             if (rc) return rc;
-            int list_nl = nl;
-            if (h->h_status->n_marked > 0) {
-                rc = remesh(h, L, nl, &list_nl);
+            if (h->h_status->n_deferred > 0) {
+                k_sweep<MODEL, D><<<blocks(nl), 256, 0, h->stream>>>(h->d, h->mc, h->d_list, nl, L, dt,
+                                                                     h->acti_counter[L], 1, h->d_status);
+                HIPCHK(h, hipGetLastError());
+                rc = read_status(h);
                 if (rc) return rc;
             }
-            k_reassign<MODEL, D><<<blocks(list_nl), 256, 0, h->stream>>>(h->d, h->mc, h->d_list, list_nl, L, dt,
-                                                                          h->acti_counter[L], h->d_status);
+            if (h->h_status->n_marked > 0) {
+                rc = remesh(h, L, nl, dt);
+                if (rc) return rc;
+            }
+            k_apply<<<blocks(nl), 256, 0, h->stream>>>(h->d, h->d_list, nl, L);
             HIPCHK(h, hipGetLastError());
             if (h->timing) {
                 HIPCHK(h, hipEventRecord(h->ev[2], h->stream));
             }
             rc = read_status(h);
             if (rc) return rc;
+            if (h->h_status->dev_error) {
+                h->err = "a neighbour subtree is more than three levels deeper than the cell (2:1 balance broken)";
+                return DACTI_E_STATE;
+            }
             if (h->timing) {
                 float a = 0, b = 0;
                 HIPCHK(h, hipEventElapsedTime(&a, h->ev[0], h->ev[1]));
@@ -364,8 +380,9 @@ struct Ops {
         return 0;
     }
 
-    // split / unsplit application and neighbour update (:97-262)
-    static int remesh(dacti_handle *h, int L, int nl, int *list_nl) {
+    // split / unsplit application and neighbour update (:97-262), then the
+    // LTS decision of the cells that were marked or created
+    static int remesh(dacti_handle *h, int L, int nl, double dt) {
         k_mark2<D><<<blocks(nl), 256, 0, h->stream>>>(h->d, h->d_list, nl, L, h->d_rn, h->d_status);
         HIPCHK(h, hipGetLastError());
         int rc = read_status(h);
@@ -396,7 +413,6 @@ struct Ops {
             changed = true;
         }
         k_unsplit<MODEL, D><<<blocks(nl), 256, 0, h->stream>>>(h->d, h->mc, h->d_list, nl, L, h->d_status);
-        k_clear_marks<<<blocks(nl), 256, 0, h->stream>>>(h->d, h->d_list, nl);
         HIPCHK(h, hipGetLastError());
         rc = read_status(h);
         if (rc) return rc;
@@ -404,10 +420,16 @@ struct Ops {
         if (changed) {
             k_neighbors<D><<<blocks(h->n), 256, 0, h->stream>>>(h->d, h->mc, h->n);
             HIPCHK(h, hipGetLastError());
-            int64_t n2 = 0;
-            rc = build_list(h, L, &n2);
-            if (rc) return rc;
-            *list_nl = (int) n2;
+        }
+        // cluster L now: the listed cells (split parents and coarsened
+        // children still in it, to be dropped), new children, unsplit parents
+        int64_t n2 = 0;
+        rc = build_list(h, L, &n2, h->d_list2);
+        if (rc) return rc;
+        if (n2 > 0) {
+            k_reassign<MODEL, D><<<blocks(n2), 256, 0, h->stream>>>(h->d, h->mc, h->d_list2, (int) n2, L, dt,
+                                                                     h->acti_counter[L], h->d_status);
+            HIPCHK(h, hipGetLastError());
         }
         return DACTI_OK;
     }
@@ -486,6 +508,7 @@ int gc(dacti_handle *h) {
     HIPCHK(h, hipMemsetAsync(nd.cluster + n_new, 0xff, (size_t) (nd.C - n_new), h->stream));
     HIPCHK(h, hipMemsetAsync(nd.buffer + n_new, 0xff, (size_t) (nd.C - n_new), h->stream));
     HIPCHK(h, hipMemsetAsync(nd.acti + n_new, 0xff, (size_t) (nd.C - n_new), h->stream));
+    nd.err = &h->d_status->dev_error;
     k_gc_remap<<<blocks(n_new), 256, 0, h->stream>>>(nd, n_new, h->sz.NFACE, keep, o2n);
     HIPCHK(h, hipGetLastError());
     HIPCHK(h, hipStreamSynchronize(h->stream));
@@ -559,7 +582,7 @@ void destroy_handle(dacti_handle *h) {
     if (!h) return;
     if (h->stream) (void) hipStreamSynchronize(h->stream);
     free_dev(h, h->d);
-    for (void *p : {(void *) h->d_list, (void *) h->d_off, (void *) h->d_rn, (void *) h->d_nsel, h->d_tmp,
+    for (void *p : {(void *) h->d_list, (void *) h->d_list2, (void *) h->d_off, (void *) h->d_rn, (void *) h->d_nsel, h->d_tmp,
                     (void *) h->d_hist, (void *) h->d_status})
         if (p) (void) hipFree(p);
     if (h->h_status) (void) hipHostFree(h->h_status);

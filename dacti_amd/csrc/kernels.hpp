@@ -18,13 +18,14 @@ namespace dacti {
 enum : uint8_t {
     F_LEAF = 1, F_ACTIVE = 2, F_GARBAGE = 4, F_CAN_SPLIT = 8, F_CAN_UNSPLIT = 16, F_SHOULD_SPLIT = 32, F_BOUNDARY = 64
 };
-enum : uint8_t { MK_SPLIT = 1, MK_UNSPLIT_CAND = 2, MK_UNSPLIT_CHILD = 4, MK_READY = 8 };
+enum : uint8_t { MK_SPLIT = 1, MK_UNSPLIT_CAND = 2, MK_UNSPLIT_CHILD = 4, MK_READY = 8, MK_DEFER = 16, MK_RA_DONE = 32 };
 
 struct Dev {
     int64_t C;
     int32_t *parent, *child_base, *nbr;     // nbr[f*C + i]
     int8_t *level, *acti, *cluster, *buffer;
     uint8_t *flags, *bc, *mark;             // bc[f*C + i]
+    int8_t *nextc;                          // cluster after this substep's LTS decision
     double *center;                         // center[d*C + i]
     double *size, *t;
     double *p, *u, *rhs, *tot;              // [e*C + i]
@@ -34,15 +35,19 @@ struct Dev {
     double *acc;                            // [(f*NF + e)*C + i]
     double *error, *ctime, *max_dt, *cfl;
     double *scratch;                        // C doubles (smoothing / reductions)
+    int32_t *err;                           // device error word (subtree deeper than supported)
 };
 
 // device status block read back by the host after a substep phase
 struct Status {
+    int32_t dev_error;      // set when a subtree is deeper than the unrolled walks
     int32_t nan_flag;
     int32_t n_marked;       // cells with a split or unsplit-candidate mark
     int32_t n_ready_split;  // split cells passing the 2:1 check
     int32_t n_new_blocks;   // of those, without a reusable child block
     int32_t n_unsplit;
+    int32_t n_deferred;
+    int32_t n_rest;         // cells left to the general flux kernel
     int32_t kept;
     int32_t lowered;
     int32_t dropped;
@@ -87,75 +92,130 @@ __device__ inline int32_t dom_find(const Dev &d, const MConst &c, double *p, int
     return idx;
 }
 
-// Domain::interpolate_p / _t / _k: integrator/domain.hpp:84-148
-template <int MODEL, int D>
-__device__ void interp_p(const Dev &d, int32_t idx, double *out) {
-    using S = Sz<MODEL, D>;
+// Domain::interpolate_p / _t / _k: integrator/domain.hpp:84-148.  The
+// reference recurses; here the recursion is unrolled to three levels below
+// the queried node (2:1 balance keeps real trees at one), and a deeper
+// subtree sets the device error word instead of recursing further.
+template <int NV, typename Leaf>
+__device__ __forceinline__ void interp_rec3(const Dev &d, int32_t idx, double *out, int nch, Leaf &&leaf) {
     if (d.flags[idx] & F_LEAF) {
-        for (int e = 0; e < S::NE; ++e) out[e] = d.p[e * d.C + idx];
+        leaf(idx, out);
         return;
     }
-    double f[S::NE];
-    int n = 0;
-    for (int e = 0; e < S::NE; ++e) f[e] = 0.0;
-    const int32_t cb = d.child_base[idx];
-    for (int ch = 0; ch < S::NCH; ++ch) {
-        if (d.flags[cb + ch] & F_ACTIVE) {
-            double g[S::NE];
-            interp_p<MODEL, D>(d, cb + ch, g);
-            for (int e = 0; e < S::NE; ++e) f[e] = f[e] + g[e];
-            n++;
+    double f1[NV];
+    int n1 = 0;
+    for (int e = 0; e < NV; ++e) f1[e] = 0.0;
+    const int32_t c1 = d.child_base[idx];
+    for (int a = 0; a < nch; ++a) {
+        const int32_t i1 = c1 + a;
+        if (!(d.flags[i1] & F_ACTIVE)) continue;
+        double g1[NV];
+        if (d.flags[i1] & F_LEAF) {
+            leaf(i1, g1);
+        } else {
+            double f2[NV];
+            int n2 = 0;
+            for (int e = 0; e < NV; ++e) f2[e] = 0.0;
+            const int32_t c2 = d.child_base[i1];
+            for (int b = 0; b < nch; ++b) {
+                const int32_t i2 = c2 + b;
+                if (!(d.flags[i2] & F_ACTIVE)) continue;
+                double g2[NV];
+                if (d.flags[i2] & F_LEAF) {
+                    leaf(i2, g2);
+                } else {
+                    double f3[NV];
+                    int n3 = 0;
+                    for (int e = 0; e < NV; ++e) f3[e] = 0.0;
+                    const int32_t c3 = d.child_base[i2];
+                    for (int c = 0; c < nch; ++c) {
+                        const int32_t i3 = c3 + c;
+                        if (!(d.flags[i3] & F_ACTIVE)) continue;
+                        double g3[NV];
+                        if (!(d.flags[i3] & F_LEAF)) atomicOr(d.err, 1);
+                        leaf(i3, g3);
+                        for (int e = 0; e < NV; ++e) f3[e] = f3[e] + g3[e];
+                        n3++;
+                    }
+                    for (int e = 0; e < NV; ++e) g2[e] = f3[e] / n3;
+                }
+                for (int e = 0; e < NV; ++e) f2[e] = f2[e] + g2[e];
+                n2++;
+            }
+            for (int e = 0; e < NV; ++e) g1[e] = f2[e] / n2;
         }
+        for (int e = 0; e < NV; ++e) f1[e] = f1[e] + g1[e];
+        n1++;
     }
-    for (int e = 0; e < S::NE; ++e) out[e] = f[e] / n;
+    for (int e = 0; e < NV; ++e) out[e] = f1[e] / n1;
+}
+
+template <int MODEL, int D>
+__device__ __noinline__ void interp_p(const Dev &d, int32_t idx, double *out) {
+    constexpr int NE = Sz<MODEL, D>::NE;
+    interp_rec3<NE>(d, idx, out, 1 << D, [&](int32_t i, double *o) {
+        for (int e = 0; e < NE; ++e) o[e] = d.p[e * d.C + i];
+    });
 }
 
 template <int D>
-__device__ double interp_t(const Dev &d, int32_t idx) {
-    if (d.flags[idx] & F_LEAF) return d.t[idx];
-    double t = 0.0;
-    int n = 0;
-    const int32_t cb = d.child_base[idx];
-    for (int ch = 0; ch < (1 << D); ++ch)
-        if (d.flags[cb + ch] & F_ACTIVE) {
-            t += interp_t<D>(d, cb + ch);
-            n++;
-        }
-    return t / n;
+__device__ __noinline__ double interp_t(const Dev &d, int32_t idx) {
+    double o[1];
+    interp_rec3<1>(d, idx, o, 1 << D, [&](int32_t i, double *r) { r[0] = d.t[i]; });
+    return o[0];
 }
 
 template <int D>
-__device__ double interp_k(const Dev &d, int32_t idx, int kk) {
-    if (d.flags[idx] & F_LEAF) return d.k[kk * d.C + idx];
-    double f = 0.0;
-    int n = 0;
-    const int32_t cb = d.child_base[idx];
-    for (int ch = 0; ch < (1 << D); ++ch)
-        if (d.flags[cb + ch] & F_ACTIVE) {
-            f += interp_k<D>(d, cb + ch, kk);
-            n++;
-        }
-    return f / n;
+__device__ __noinline__ double interp_k(const Dev &d, int32_t idx, int kk) {
+    double o[1];
+    interp_rec3<1>(d, idx, o, 1 << D, [&](int32_t i, double *r) { r[0] = d.k[kk * d.C + i]; });
+    return o[0];
 }
 
 // Active leaves of subtree `root` touching its face (dim, side), visited in
-// depth-first child order.
+// depth-first child order, down to three levels below root (deeper
+// subtrees set the device error word).
 template <int D, typename Fn>
 __device__ inline void for_sub_face_leaves(const Dev &d, int32_t root, int dim, int side, Fn &&fn) {
-    int32_t stack[48];
-    int sp = 0;
-    stack[sp++] = root;
-    while (sp > 0) {
-        const int32_t idx = stack[--sp];
+    auto visit_leaf = [&](int32_t idx) {
         const uint8_t fl = d.flags[idx];
-        if (fl & F_LEAF) {
-            if ((fl & F_ACTIVE) && !(fl & F_GARBAGE)) fn(idx);
+        if ((fl & F_ACTIVE) && !(fl & F_GARBAGE)) fn(idx);
+    };
+    if (d.flags[root] & F_LEAF) {
+        visit_leaf(root);
+        return;
+    }
+    const int32_t c1 = d.child_base[root];
+    for (int a = 0; a < (1 << D); ++a) {
+        if (((a >> dim) & 1) != side) continue;
+        const int32_t i1 = c1 + a;
+        if (d.flags[i1] & F_LEAF) {
+            visit_leaf(i1);
             continue;
         }
-        const int32_t cb = d.child_base[idx];
-        for (int ch = (1 << D) - 1; ch >= 0; --ch)
-            if (((ch >> dim) & 1) == side && sp < 48) stack[sp++] = cb + ch;
+        const int32_t c2 = d.child_base[i1];
+        for (int b = 0; b < (1 << D); ++b) {
+            if (((b >> dim) & 1) != side) continue;
+            const int32_t i2 = c2 + b;
+            if (d.flags[i2] & F_LEAF) {
+                visit_leaf(i2);
+                continue;
+            }
+            const int32_t c3 = d.child_base[i2];
+            for (int c = 0; c < (1 << D); ++c) {
+                if (((c >> dim) & 1) != side) continue;
+                const int32_t i3 = c3 + c;
+                if (!(d.flags[i3] & F_LEAF)) atomicOr(d.err, 1);
+                visit_leaf(i3);
+            }
+        }
     }
+}
+
+// count of lanes with pred, added once per wavefront
+__device__ __forceinline__ void wave_count_add(bool pred, int32_t *counter) {
+    const unsigned long long m = __ballot(pred);
+    if (m && (threadIdx.x & 63) == (unsigned) (__ffsll((long long) m) - 1)) atomicAdd(counter, (int32_t) __popcll(m));
 }
 
 // Cell data one side of a face needs (integrator/cell.hpp:8-38)
@@ -260,21 +320,18 @@ __device__ inline void model_flux(const MConst &c, const Side<MODEL, D> &n, doub
     }
 }
 
-// The owner x computes its face (dim, dir): impl/flux.hpp:32-91.  Returns F
-// with the reference's sign convention (x receives -F if dir==1, +F if
-// dir==0; the neighbour the opposite).
+// Flux of owner x's face (dim, dir) from the two loaded sides: impl/flux.hpp:32-91.
+// sx is x's side, sn the neighbour's; nbc the neighbour centre along dim.
+// Returns F with the reference's sign convention (x receives -F if dir==1,
+// +F if dir==0; the neighbour the opposite).
 template <int MODEL, int D>
-__device__ inline void owner_face_flux(const Dev &d, const MConst &c, int32_t x, int dim, int dir, double dt,
+__device__ inline void face_flux_sides(const MConst &c, const Side<MODEL, D> &sx, double xc, double xs, int xlev,
+                                       const Side<MODEL, D> &sn, double nbc, int dim, int dir, double dt,
                                        double *F) {
-    const int64_t C = d.C;
-    const int32_t nb = d.nbr[(2 * dim + dir) * C + x];
-    const double xc = d.center[dim * C + x];
-    const double xs = d.size[x];
     const double fc = xc + xs * (dir - 0.5);
-    const double t0 = d.t[x];
+    const double t0 = sx.t;
     const double t1 = t0 + dt;
-    const double area = c.area[d.level[x]];
-    double nbc = d.center[dim * C + nb];
+    const double area = c.area[xlev];
     if (c.periodic[dim]) {
         if (dir == 1) {
             if (xc + xs > 0.5 * c.active_size[dim]) nbc += c.active_size[dim];
@@ -282,11 +339,19 @@ __device__ inline void owner_face_flux(const Dev &d, const MConst &c, int32_t x,
             if (xc - xs < -0.5 * c.active_size[dim]) nbc -= c.active_size[dim];
         }
     }
-    Side<MODEL, D> sx, sn;
-    load_side<MODEL, D>(d, x, dim, sx);
-    load_side<MODEL, D>(d, nb, dim, sn);
     if (dir == 1) model_flux<MODEL, D>(c, sx, xc, sn, nbc, fc, area, t0, t1, dim, F);
     else model_flux<MODEL, D>(c, sn, nbc, sx, xc, fc, area, t0, t1, dim, F);
+}
+
+// Same, loading x's side from memory; sn is x's neighbour across the face.
+template <int MODEL, int D>
+__device__ inline void owner_face_flux_with(const Dev &d, const MConst &c, int32_t x, const Side<MODEL, D> &sn,
+                                            int32_t nb, int dim, int dir, double dt, double *F) {
+    const int64_t C = d.C;
+    Side<MODEL, D> sx;
+    load_side<MODEL, D>(d, x, dim, sx);
+    face_flux_sides<MODEL, D>(c, sx, d.center[dim * C + x], d.size[x], d.level[x], sn, d.center[dim * C + nb], dim,
+                              dir, dt, F);
 }
 
 // boundary_flux: integrator/boundary.hpp:130-160 (already scaled)
@@ -332,21 +397,39 @@ __device__ __forceinline__ int bc_face(const Dev &d, int32_t y, int f) {
 // ------------------------------------------------------------------ flux
 // One level-L substep's flux totals: impl/flux.hpp:10-97 (restated as a
 // gather, see the file comment).  Writes tot[NF] of every listed cell and
-// deposits fluxes owed to cells outside the cluster.
+// deposits fluxes owed to cells outside the cluster.  The listed cell's own
+// data is loaded once; one neighbour side is loaded per face.
 template <int MODEL, int D>
-__global__ void __launch_bounds__(256) k_flux(Dev d, MConst c, const int32_t *__restrict__ list, int nl, int L,
-                                               double dt) {
+__device__ void flux_cell(const Dev &d, const MConst &c, int32_t y, int L, double dt) {
     using S = Sz<MODEL, D>;
-    constexpr int NF = S::NF;
-    const int li = blockIdx.x * blockDim.x + threadIdx.x;
-    if (li >= nl) return;
+    constexpr int NF = S::NF, NE = S::NE;
     const int64_t C = d.C;
-    const int32_t y = list[li];
     const int ylev = d.level[y];
+    const double ys = d.size[y];
+    Side<MODEL, D> sy;
+#pragma unroll
+    for (int e = 0; e < NE; ++e) {
+        sy.p[e] = d.p[e * C + y];
+        sy.r[e] = d.rhs[e * C + y];
+    }
+    sy.t = d.t[y];
+    if constexpr (MODEL == MULTI) sy.kap = d.k[S::IKAPPA * C + y];
+    if constexpr (S::NG > 0) {
+#pragma unroll
+        for (int a = 0; a < D; ++a)
+#pragma unroll
+            for (int e = 0; e < S::NG; ++e) sy.g[a][e] = d.grad[(a * S::NGX + e) * C + y];
+    }
     double tot[NF];
 #pragma unroll
     for (int e = 0; e < NF; ++e) tot[e] = 0.0;
+#pragma unroll 1
     for (int dim = 0; dim < D; ++dim) {
+#pragma unroll
+        for (int e = 0; e < NE; ++e) sy.s[e] = d.slope[(dim * NE + e) * C + y];
+        if constexpr (MODEL == MULTI) sy.kn = d.k[(S::IKN + dim) * C + y];
+        const double yc = d.center[dim * C + y];
+#pragma unroll 1
         for (int dir = 0; dir < 2; ++dir) {
             const int f = 2 * dim + dir;
             const int32_t nb = d.nbr[f * C + y];
@@ -377,11 +460,20 @@ __global__ void __launch_bounds__(256) k_flux(Dev d, MConst c, const int32_t *__
                     y_computes = nacti <= L && (dir == 1 || nacti < L);
                     nb_computes = ncl == L && dir == 0 && d.nbr[(2 * dim + 1) * C + nb] == y;
                 }
-                if (y_computes) {
-                    owner_face_flux<MODEL, D>(d, c, y, dim, dir, dt, F);
+                if (y_computes || nb_computes) {
+                    Side<MODEL, D> sn;
+                    load_side<MODEL, D>(d, nb, dim, sn);
+                    const double nbc = d.center[dim * C + nb];
+                    if (y_computes) {
+                        face_flux_sides<MODEL, D>(c, sy, yc, ys, ylev, sn, nbc, dim, dir, dt, F);
 #pragma unroll
-                    for (int e = 0; e < NF; ++e) cc[e] = dir == 1 ? cc[e] - F[e] : cc[e] + F[e];
-                    if (ncl != L) {
+                        for (int e = 0; e < NF; ++e) cc[e] = dir == 1 ? cc[e] - F[e] : cc[e] + F[e];
+                    } else {
+                        face_flux_sides<MODEL, D>(c, sn, nbc, d.size[nb], nlev, sy, yc, dim, 1, dt, F);
+#pragma unroll
+                        for (int e = 0; e < NF; ++e) cc[e] = cc[e] + F[e];
+                    }
+                    if (y_computes && ncl != L) {
                         const int fo = 2 * dim + (1 - dir);
                         if (nlev == ylev) {
 #pragma unroll
@@ -401,7 +493,7 @@ __global__ void __launch_bounds__(256) k_flux(Dev d, MConst c, const int32_t *__
                                 for_sub_face_leaves<D>(d, rn, dim, dir, [&](int32_t x) {
                                     if (d.cluster[x] == L && d.nbr[f * C + x] == nb) {
                                         double G[NF];
-                                        owner_face_flux<MODEL, D>(d, c, x, dim, dir, dt, G);
+                                        owner_face_flux_with<MODEL, D>(d, c, x, sn, nb, dim, dir, dt, G);
                                         for (int e = 0; e < NF; ++e) {
                                             double &a = d.acc[(fo * NF + e) * C + nb];
                                             a = dir == 1 ? a + G[e] : a - G[e];
@@ -411,10 +503,6 @@ __global__ void __launch_bounds__(256) k_flux(Dev d, MConst c, const int32_t *__
                             }
                         }
                     }
-                } else if (nb_computes) {
-                    owner_face_flux<MODEL, D>(d, c, nb, dim, 1, dt, F);
-#pragma unroll
-                    for (int e = 0; e < NF; ++e) cc[e] = cc[e] + F[e];
                 }
             } else {
                 // finer neighbours: received deposits plus this substep's sub-faces
@@ -432,10 +520,135 @@ __global__ void __launch_bounds__(256) k_flux(Dev d, MConst c, const int32_t *__
                 for_sub_face_leaves<D>(d, nb, dim, 1 - dir, [&](int32_t x) {
                     if (d.cluster[x] == L && d.nbr[(2 * dim + (1 - dir)) * C + x] == y) {
                         double G[NF];
-                        owner_face_flux<MODEL, D>(d, c, x, dim, 1 - dir, dt, G);
+                        owner_face_flux_with<MODEL, D>(d, c, x, sy, y, dim, 1 - dir, dt, G);
                         for (int e = 0; e < NF; ++e) cc[e] = (1 - dir) == 1 ? cc[e] + G[e] : cc[e] - G[e];
                     }
                 });
+            }
+#pragma unroll
+            for (int e = 0; e < NF; ++e) tot[e] = tot[e] + cc[e];
+        }
+    }
+#pragma unroll
+    for (int e = 0; e < NF; ++e) d.tot[e * C + y] = tot[e];
+}
+
+// general flux kernel over the cells the lean kernel set aside (count in *n)
+template <int MODEL, int D>
+__global__ void __launch_bounds__(256) k_flux(Dev d, MConst c, const int32_t *__restrict__ list,
+                                               const int32_t *__restrict__ n, int L, double dt) {
+    const int nl = *n;
+    for (int li = blockIdx.x * blockDim.x + threadIdx.x; li < nl; li += gridDim.x * blockDim.x)
+        flux_cell<MODEL, D>(d, c, list[li], L, dt);
+}
+
+// Lean flux kernel for cells whose faces are all domain boundaries or
+// same-tree-level leaves (every cell of a uniform tree, most cells of an
+// adaptive one): one flux evaluation site per face, no sub-face walks.
+// A cell with any other face is appended to `rest` (order irrelevant:
+// cells are independent) and handled by k_flux.
+template <int MODEL, int D>
+__global__ void __launch_bounds__(256) k_flux_simple(Dev d, MConst c, const int32_t *__restrict__ list, int nl,
+                                                      int L, double dt, int32_t *rest, int32_t *n_rest) {
+    using S = Sz<MODEL, D>;
+    constexpr int NF = S::NF, NE = S::NE;
+    const int li = blockIdx.x * blockDim.x + threadIdx.x;
+    if (li >= nl) return;
+    const int64_t C = d.C;
+    const int32_t y = list[li];
+    const int ylev = d.level[y];
+    // classify: every face a boundary or a same-level leaf?
+    bool simple = true;
+    for (int f = 0; f < 2 * D; ++f) {
+        const int32_t nb = d.nbr[f * C + y];
+        if (nb != 0 && (!(d.flags[nb] & F_LEAF) || d.level[nb] != ylev)) simple = false;
+    }
+    if (!simple) {
+        rest[atomicAdd(n_rest, 1)] = y;
+        return;
+    }
+    const double ys = d.size[y];
+    const double area = c.area[ylev];
+    Side<MODEL, D> sy;
+#pragma unroll
+    for (int e = 0; e < NE; ++e) {
+        sy.p[e] = d.p[e * C + y];
+        sy.r[e] = d.rhs[e * C + y];
+    }
+    sy.t = d.t[y];
+    if constexpr (MODEL == MULTI) sy.kap = d.k[S::IKAPPA * C + y];
+    if constexpr (S::NG > 0) {
+#pragma unroll
+        for (int a = 0; a < D; ++a)
+#pragma unroll
+            for (int e = 0; e < S::NG; ++e) sy.g[a][e] = d.grad[(a * S::NGX + e) * C + y];
+    }
+    double tot[NF];
+#pragma unroll
+    for (int e = 0; e < NF; ++e) tot[e] = 0.0;
+#pragma unroll 1
+    for (int dim = 0; dim < D; ++dim) {
+#pragma unroll
+        for (int e = 0; e < NE; ++e) sy.s[e] = d.slope[(dim * NE + e) * C + y];
+        if constexpr (MODEL == MULTI) sy.kn = d.k[(S::IKN + dim) * C + y];
+        const double yc = d.center[dim * C + y];
+#pragma unroll 1
+        for (int dir = 0; dir < 2; ++dir) {
+            const int f = 2 * dim + dir;
+            const int32_t nb = d.nbr[f * C + y];
+            double cc[NF], F[NF];
+#pragma unroll
+            for (int e = 0; e < NF; ++e) cc[e] = 0.0;
+            if (nb == 0) {
+                boundary_flux<MODEL, D>(d, c, y, bc_face(d, y, f), dim, dir, dt, F);
+#pragma unroll
+                for (int e = 0; e < NF; ++e) cc[e] = cc[e] - F[e];
+            } else {
+                const int nacti = d.acti[nb];
+                const int ncl = d.cluster[nb];
+                if (nacti > L) {
+#pragma unroll
+                    for (int e = 0; e < NF; ++e) {
+                        cc[e] = d.acc[(f * NF + e) * C + y];
+                        d.acc[(f * NF + e) * C + y] = 0.0;
+                    }
+                }
+                const bool y_computes = nacti <= L && (dir == 1 || nacti < L);
+                const bool nb_computes = !y_computes && ncl == L && dir == 0 && d.nbr[(2 * dim + 1) * C + nb] == y;
+                if (y_computes || nb_computes) {
+                    Side<MODEL, D> sn;
+                    load_side<MODEL, D>(d, nb, dim, sn);
+                    const double nbc = d.center[dim * C + nb];
+                    // the owner's face centre, time interval and periodic image
+                    const double oc = y_computes ? yc : nbc;
+                    const int odir = y_computes ? dir : 1;
+                    const double fc = oc + ys * (odir - 0.5);
+                    const double t0 = y_computes ? sy.t : sn.t;
+                    const double t1 = t0 + dt;
+                    double other = y_computes ? nbc : yc;
+                    if (c.periodic[dim]) {
+                        if (odir == 1) {
+                            if (oc + ys > 0.5 * c.active_size[dim]) other += c.active_size[dim];
+                        } else {
+                            if (oc - ys < -0.5 * c.active_size[dim]) other -= c.active_size[dim];
+                        }
+                    }
+                    const double cy = y_computes ? yc : other;
+                    const double cn = y_computes ? other : nbc;
+                    if (dir == 1) model_flux<MODEL, D>(c, sy, cy, sn, cn, fc, area, t0, t1, dim, F);
+                    else model_flux<MODEL, D>(c, sn, cn, sy, cy, fc, area, t0, t1, dim, F);
+                    // y receives -F on its + face and +F on its - face
+#pragma unroll
+                    for (int e = 0; e < NF; ++e) cc[e] = dir == 1 ? cc[e] - F[e] : cc[e] + F[e];
+                    if (y_computes && ncl != L) {
+                        const int fo = 2 * dim + (1 - dir);
+#pragma unroll
+                        for (int e = 0; e < NF; ++e) {
+                            double &a = d.acc[(fo * NF + e) * C + nb];
+                            a = dir == 1 ? a + F[e] : a - F[e];
+                        }
+                    }
+                }
             }
 #pragma unroll
             for (int e = 0; e < NF; ++e) tot[e] = tot[e] + cc[e];
@@ -575,6 +788,207 @@ __global__ void __launch_bounds__(256) k_slope(Dev d, MConst c, const int32_t *_
     const int li = blockIdx.x * blockDim.x + threadIdx.x;
     if (li >= nl) return;
     compute_slope_dev<MODEL, D>(d, c, list[li]);
+}
+
+// Scene::can_split: scene/scene.hpp:50-61
+template <int D>
+__device__ inline bool scene_can_split(const MConst &c, const double *x, int level) {
+    if (level < c.max_refinement_level) return true;
+    for (int z = 0; z < c.n_zones; ++z) {
+        const double x3[3] = {x[0], D > 1 ? x[1 % D] : 0.0, D > 2 ? x[2 % D] : 0.0};
+        if (level < c.zone_level[z] && x3[0] < c.zone_x1[z][0] && x3[1] < c.zone_x1[z][1] && x3[2] < c.zone_x1[z][2] &&
+            x3[0] > c.zone_x0[z][0] && x3[1] > c.zone_x0[z][1] && x3[2] > c.zone_x0[z][2])
+            return true;
+    }
+    return false;
+}
+
+// ------------------------------------------------------------ sweep
+// After the update of level L: slopes (impl/slopes.hpp:15-76), the error
+// estimate and split / unsplit marks (impl/adaptive_refinement.hpp:18-94)
+// and, for unmarked cells, the LTS decision (:264-322) in one neighbour
+// sweep.  The decision is stored (nextc, buffer) and applied by k_apply
+// after any remesh, so the remesh sees the levels the reference sees.  A
+// cell whose error needs the new slope of a coarser neighbour updated in
+// the same substep is deferred to a second pass (pass = 1).
+template <int MODEL, int D>
+__device__ inline void lts_decide(const Dev &d, const MConst &c, int32_t y, const double *p, double size, int L,
+                                  double dt, int counter, bool &lowered) {
+    const double mdt = max_delta_time<MODEL, D>(c, p, size);
+    double cfl = dt / mdt;
+    int8_t nc = (int8_t) L;
+    if (cfl < 0.5 * c.max_cfl && counter == 1 && L > 0) {
+        d.buffer[y] = (int8_t) (L - 1);
+        nc = -1;
+        cfl *= 2;
+        lowered = true;
+    } else if (cfl > c.max_cfl && L < 15) {
+        const int raise = (int) (log2(cfl / c.max_cfl) + 1);
+        if (L + raise < 16) {
+            nc = (int8_t) (L + raise);
+            cfl /= 1 << raise;
+        } else {
+            nc = -1;
+        }
+    }
+    d.max_dt[y] = mdt;
+    d.cfl[y] = cfl;
+    d.nextc[y] = nc;
+}
+
+template <int MODEL, int D>
+__global__ void __launch_bounds__(256) k_sweep(Dev d, MConst c, const int32_t *__restrict__ list, int nl, int L,
+                                                double dt, int counter, int pass, Status *st) {
+    using S = Sz<MODEL, D>;
+    constexpr int NE = S::NE;
+    const int li = blockIdx.x * blockDim.x + threadIdx.x;
+    if (li >= nl) return;
+    const int64_t C = d.C;
+    const int32_t y = list[li];
+    if (pass == 1 && !(d.mark[y] & MK_DEFER)) return;
+    double pc[NE], ctr[D], rhs[NE];
+    for (int e = 0; e < NE; ++e) pc[e] = d.p[e * C + y];
+    for (int a = 0; a < D; ++a) ctr[a] = d.center[a * C + y];
+    const double ys = d.size[y], yt = d.t[y];
+    const int ylev = d.level[y];
+    const double kap = MODEL == MULTI ? d.k[S::IKAPPA * C + y] : 0.0;
+    double a_ip_in, a_in_ip, a_in_ia;
+    rhs_coeffs<MODEL, D>(c, pc, kap, a_ip_in, a_in_ip, a_in_ia);
+    for (int e = 0; e < NE; ++e) rhs[e] = 0.0;
+    double ts[S::NGX];
+    if constexpr (S::NG > 0)
+        for (int e = 0; e < S::NG; ++e) ts[e] = d.t_slope[e * C + y];
+    bool deferred = false;
+    double err = 0.0;
+#pragma unroll 1
+    for (int dim = 0; dim < D; ++dim) {
+        double pn[NE], pp[NE], qn[NE], qp[NE], xn, xp, tn, tp;
+        bool hn, hp;
+        const double xc = 0.0;
+        for (int dir = 0; dir < 2; ++dir) {
+            const int32_t nb = d.nbr[(2 * dim + dir) * C + y];
+            double *pv = dir ? pp : pn;
+            double *qv = dir ? qp : qn;
+            double tv, xv;
+            bool hv = true;
+            if (nb == 0) {
+                boundary_value<MODEL, D>(c, bc_face(d, y, 2 * dim + dir), pc, yt, pv);
+                xv = dir ? ys : -ys;
+                tv = yt;
+                hv = false;
+            } else {
+                if (!(d.flags[nb] & F_LEAF)) {
+                    interp_p<MODEL, D>(d, nb, pv);
+                    tv = interp_t<D>(d, nb);
+                    for (int e = 0; e < NE; ++e) qv[e] = pv[e];
+                } else {
+                    for (int e = 0; e < NE; ++e) pv[e] = d.p[e * C + nb];
+                    tv = d.t[nb];
+                    if (d.level[nb] < ylev) {
+                        if (pass == 0 && d.cluster[nb] == L) {
+                            deferred = true;
+                        } else {
+                            // the coarser leaf's linear reconstruction at the
+                            // neighbour position (adaptive_refinement.hpp:48-59)
+                            double x[D];
+                            for (int a = 0; a < D; ++a) x[a] = ctr[a];
+                            x[dim] = ctr[dim] + ys * (2 * dir - 1.0);
+                            if (c.periodic[dim] && dir == 0 && x[dim] < -0.5 * c.active_size[dim])
+                                x[dim] += c.active_size[dim];
+                            if (c.periodic[dim] && dir == 1 && x[dim] > 0.5 * c.active_size[dim])
+                                x[dim] -= c.active_size[dim];
+                            for (int e = 0; e < NE; ++e) qv[e] = pv[e];
+                            for (int a = 0; a < D; ++a) {
+                                const double dx = x[a] - d.center[a * C + nb];
+                                for (int e = 0; e < NE; ++e) qv[e] = qv[e] + d.slope[(a * NE + e) * C + nb] * dx;
+                            }
+                        }
+                    } else {
+                        for (int e = 0; e < NE; ++e) qv[e] = pv[e];
+                    }
+                }
+                xv = dir ? 0.5 * (ys + d.size[nb]) : -0.5 * (ys + d.size[nb]);
+            }
+            if (dir) {
+                xp = xv;
+                tp = tv;
+                hp = hv;
+            } else {
+                xn = xv;
+                tn = tv;
+                hn = hv;
+            }
+        }
+        double sl[NE];
+        for (int e = 0; e < NE; ++e) {
+            const double sn = (pc[e] - pn[e]) / (xc - xn);
+            const double sp = (pp[e] - pc[e]) / (xp - xc);
+            sl[e] = limiter(c.limiter, sn, sp);
+        }
+        if (pass == 0) {
+            // rhs[i] gains the dim-j terms in the reference's (i, j, k) order
+            rhs_add_dim<MODEL, D>(c, pc, sl, dim, kap, a_ip_in, a_in_ip, a_in_ia, rhs);
+            for (int e = 0; e < NE; ++e) d.slope[(dim * NE + e) * C + y] = sl[e];
+            if constexpr (S::NG > 0) {
+                double gp[S::NG], gn[S::NG];
+                grad_input<MODEL, D>(c, pp, gp);
+                grad_input<MODEL, D>(c, pn, gn);
+                for (int e = 0; e < S::NG; ++e)
+                    d.grad[(dim * S::NGX + e) * C + y] = (gp[e] - gn[e] - ts[e] * (tp - tn)) / (xp - xn);
+            }
+        }
+        // weighted_laplacian::estimate, error_estimators.hpp:19-37
+        double sq = 0.0;
+        for (int e = 0; e < NE; ++e) {
+            double d2 = pc[e] * -2.0;
+            d2 = hn ? d2 + qn[e] : 0.0;
+            d2 = (hn && hp) ? d2 + qp[e] : 0.0;
+            const double v = d2 * c.weights[e];
+            sq += v * v;
+        }
+        err += sq;
+    }
+    if (pass == 0)
+        for (int e = 0; e < NE; ++e) d.rhs[e * C + y] = rhs[e];
+    bool lowered = false, marked = false;
+    if (deferred) {
+        d.mark[y] = MK_DEFER;
+    } else {
+        err = sqrt(err);
+        d.error[y] = err;
+        uint8_t fl = d.flags[y];
+        if (fl & F_CAN_SPLIT) {
+            if (!scene_can_split<D>(c, ctr, ylev)) fl &= ~F_CAN_SPLIT;
+            d.flags[y] = fl;
+        }
+        uint8_t m = 0;
+        if ((err > c.max_error && (fl & F_CAN_SPLIT)) || (fl & F_SHOULD_SPLIT)) m |= MK_SPLIT;
+        if (err < c.min_error && (fl & F_CAN_UNSPLIT) && ylev > 0 && counter == 1) m |= MK_UNSPLIT_CAND;
+        if (m == 0) {
+            lts_decide<MODEL, D>(d, c, y, pc, ys, L, dt, counter, lowered);
+            m = MK_RA_DONE;
+        } else {
+            marked = true;
+        }
+        d.mark[y] = m;
+    }
+    wave_count_add(deferred, &st->n_deferred);
+    wave_count_add(marked, &st->n_marked);
+    wave_count_add(lowered, &st->lowered);
+}
+
+// apply the stored LTS decisions (impl/adaptive_refinement.hpp:308-321) and
+// clear the marks of the listed cells
+__global__ void k_apply(Dev d, const int32_t *__restrict__ list, int nl, int L) {
+    const int li = blockIdx.x * blockDim.x + threadIdx.x;
+    if (li >= nl) return;
+    const int32_t y = list[li];
+    if (d.mark[y] & MK_RA_DONE) {
+        const int8_t nc = d.nextc[y];
+        d.cluster[y] = nc;
+        if (nc > L) d.acti[y] = nc;
+    }
+    d.mark[y] = 0;
 }
 
 // ------------------------------------------------- multiphase interface
@@ -734,19 +1148,6 @@ __global__ void k_smooth_curvature(Dev d, MConst c, const int32_t *__restrict__ 
 }
 
 // --------------------------------------------------- refinement flagging
-// Scene::can_split: scene/scene.hpp:50-61
-template <int D>
-__device__ inline bool scene_can_split(const MConst &c, const double *x, int level) {
-    if (level < c.max_refinement_level) return true;
-    for (int z = 0; z < c.n_zones; ++z) {
-        const double x3[3] = {x[0], D > 1 ? x[1 % D] : 0.0, D > 2 ? x[2 % D] : 0.0};
-        if (level < c.zone_level[z] && x3[0] < c.zone_x1[z][0] && x3[1] < c.zone_x1[z][1] && x3[2] < c.zone_x1[z][2] &&
-            x3[0] > c.zone_x0[z][0] && x3[1] > c.zone_x0[z][1] && x3[2] > c.zone_x0[z][2])
-            return true;
-    }
-    return false;
-}
-
 // error estimate + split / unsplit marks: impl/adaptive_refinement.hpp:18-94,
 // weighted_laplacian::estimate error_estimators.hpp:19-37
 template <int MODEL, int D>
@@ -1070,42 +1471,39 @@ template <int MODEL, int D>
 __global__ void __launch_bounds__(256) k_reassign(Dev d, MConst c, const int32_t *__restrict__ list, int nl, int L,
                                                    double dt, int counter, Status *st) {
     using S = Sz<MODEL, D>;
-    const int li = blockIdx.x * blockDim.x + threadIdx.x;
-    if (li >= nl) return;
+    const int li0 = blockIdx.x * blockDim.x + threadIdx.x;
+    if (li0 - (int) (threadIdx.x & 63) >= nl) return;  // whole wavefront past the end
+    const bool live = li0 < nl;
+    const int li = live ? li0 : nl - 1;
     const int64_t C = d.C;
     const int32_t y = list[li];
+    if (!live || (d.mark[y] & MK_RA_DONE)) return;
     double p[S::NE];
     for (int e = 0; e < S::NE; ++e) p[e] = d.p[e * C + y];
     const double mdt = max_delta_time<MODEL, D>(c, p, d.size[y]);
     double cfl = dt / mdt;
     d.max_dt[y] = mdt;
     const uint8_t fl = d.flags[y];
+    bool lowered = false;
     if (!(fl & F_LEAF) || (fl & F_GARBAGE)) {
-        d.cfl[y] = cfl;
         d.cluster[y] = -1;
-        atomicAdd(&st->dropped, 1);
-        return;
-    }
-    if (cfl < 0.5 * c.max_cfl && counter == 1 && L > 0) {
+    } else if (cfl < 0.5 * c.max_cfl && counter == 1 && L > 0) {
         d.cluster[y] = -1;
         d.buffer[y] = (int8_t) (L - 1);
         cfl *= 2;
-        atomicAdd(&st->lowered, 1);
+        lowered = true;
     } else if (cfl > c.max_cfl && L < 15) {
         const int raise = (int) (log2(cfl / c.max_cfl) + 1);
         if (L + raise < 16) {
             d.acti[y] = (int8_t) (L + raise);
             cfl /= 1 << raise;
             d.cluster[y] = (int8_t) (L + raise);
-            atomicAdd(&st->raised[L + raise], 1);
         } else {
             d.cluster[y] = -1;
-            atomicAdd(&st->dropped, 1);
         }
-    } else {
-        atomicAdd(&st->kept, 1);
     }
     d.cfl[y] = cfl;
+    wave_count_add(lowered, &st->lowered);
 }
 
 // flush_acti_buffer: impl/acti_utils.hpp:43-50

@@ -176,62 +176,58 @@ __device__ inline void flux_F(const MConst &c, const double *p, double up, int d
 // rhs = sum_j A_j(p) slope_j (Hancock time derivative): euler.hpp:86-116,
 // multi.hpp:126-169.  Only the structurally non-zero Jacobian entries are
 // summed, in the reference's (i, j, k) order; the skipped products are
-// zero and leave the running sum unchanged.
+// zero and leave the running sum unchanged.  rhs_add_dim adds the terms of
+// one dimension j to every row, so a caller can stream the slopes.
 template <int MODEL, int D>
-__device__ inline void compute_rhs(const MConst &c, const double *p, const double (*slope)[Sz<MODEL, D>::NE],
-                                   double kappa, double *rhs) {
+__device__ inline void rhs_coeffs(const MConst &c, const double *p, double kappa, double &a_ip_in, double &a_in_ip,
+                                  double &a_in_ia) {
     using S = Sz<MODEL, D>;
-    constexpr int NE = S::NE, IU = S::IU, IP = S::IP;
-    double a_ip_in, a_in_ip, a_in_ia = 0.0;
+    a_in_ia = 0.0;
     if constexpr (MODEL == MULTI) {
-        a_ip_in = (p[0] + p[1]) * msound_speed_2(c, p[0], p[1], p[IP], p[S::IA]);
+        a_ip_in = (p[0] + p[1]) * msound_speed_2(c, p[0], p[1], p[S::IP], p[S::IA]);
         a_in_ip = 1.0 / (p[0] + p[1]);
         a_in_ia = c.sigma * kappa / (p[0] + p[1]);
     } else {
-        a_ip_in = p[0] * (c.gamma * p[IP] / p[0]);
+        a_ip_in = p[0] * (c.gamma * p[S::IP] / p[0]);
         a_in_ip = 1.0 / p[0];
     }
+}
+
+template <int MODEL, int D>
+__device__ inline void rhs_add_dim(const MConst &c, const double *p, const double *sl, int j, double kappa,
+                                   double a_ip_in, double a_in_ip, double a_in_ia, double *rhs) {
+    using S = Sz<MODEL, D>;
+    constexpr int NE = S::NE, IU = S::IU, IP = S::IP;
+    const int in = IU + j;
     for (int i = 0; i < NE; ++i) {
-        double r = 0.0;
-        for (int j = 0; j < D; ++j) {
-            const int in = IU + j;
-            // row i of A_j, columns in increasing k order
-            // diagonal entry p[in] at k == i; extra entries at
-            //   (0,in)=p0, (1,in)=p1 (multi), (IP,in), (in,IP), (in,IA) (multi)
-            if constexpr (MODEL == MULTI) {
-                if (i == 0) {
-                    r += p[in] * slope[j][0];
-                    r += p[0] * slope[j][in];
-                } else if (i == 1) {
-                    r += p[in] * slope[j][1];
-                    r += p[1] * slope[j][in];
-                } else if (i == in) {
-                    r += p[in] * slope[j][in];
-                    r += a_in_ip * slope[j][IP];
-                    r += a_in_ia * slope[j][S::IA];
-                } else if (i == IP) {
-                    r += a_ip_in * slope[j][in];
-                    r += p[in] * slope[j][IP];
-                } else {
-                    r += p[in] * slope[j][i];
-                }
-            } else {
-                if (i == 0) {
-                    r += p[in] * slope[j][0];
-                    r += p[0] * slope[j][in];
-                } else if (i == in) {
-                    r += p[in] * slope[j][in];
-                    r += a_in_ip * slope[j][IP];
-                } else if (i == IP) {
-                    r += a_ip_in * slope[j][in];
-                    r += p[in] * slope[j][IP];
-                } else {
-                    r += p[in] * slope[j][i];
-                }
-            }
+        double r = rhs[i];
+        if (i == 0) {
+            r += p[in] * sl[0];
+            r += p[0] * sl[in];
+        } else if (MODEL == MULTI && i == 1) {
+            r += p[in] * sl[1];
+            r += p[1] * sl[in];
+        } else if (i == in) {
+            r += p[in] * sl[in];
+            r += a_in_ip * sl[IP];
+            if constexpr (MODEL == MULTI) r += a_in_ia * sl[S::IA];
+        } else if (i == IP) {
+            r += a_ip_in * sl[in];
+            r += p[in] * sl[IP];
+        } else {
+            r += p[in] * sl[i];
         }
         rhs[i] = r;
     }
+}
+
+template <int MODEL, int D>
+__device__ inline void compute_rhs(const MConst &c, const double *p, const double (*slope)[Sz<MODEL, D>::NE],
+                                   double kappa, double *rhs) {
+    double a_ip_in, a_in_ip, a_in_ia;
+    rhs_coeffs<MODEL, D>(c, p, kappa, a_ip_in, a_in_ip, a_in_ia);
+    for (int i = 0; i < Sz<MODEL, D>::NE; ++i) rhs[i] = 0.0;
+    for (int j = 0; j < D; ++j) rhs_add_dim<MODEL, D>(c, p, slope[j], j, kappa, a_ip_in, a_in_ip, a_in_ia, rhs);
 }
 
 // HLLC-type star state in primitives: euler.hpp:118-153, multi.hpp:205-246
