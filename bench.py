@@ -40,6 +40,23 @@ def cpu_baseline(case, tree, seconds_target=20.0):
                       f"({n} leaf-cell updates in {el:.1f} s, OpenMP {threads} threads)"}
 
 
+def copy_probe(gib=1.0):
+    """A plain device-to-device copy of known size (HIP runtime), run in the
+    same process so hardware byte counters can be calibrated against it."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    n = int(gib * (1 << 30))
+    a, b = C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(a), C.c_size_t(n)) == 0 and hip.hipMalloc(C.byref(b), C.c_size_t(n)) == 0
+    hip.hipMemset(a, 1, C.c_size_t(n))
+    hip.hipDeviceSynchronize()
+    hip.hipMemcpy(b, a, C.c_size_t(n), 3)  # hipMemcpyDeviceToDevice
+    hip.hipDeviceSynchronize()
+    hip.hipFree(a)
+    hip.hipFree(b)
+    print(f"[bench] copy probe: {n} bytes device to device", file=sys.stderr, flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -47,6 +64,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--level", type=int, default=8)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-probe", action="store_true", help="run a 1 GiB device copy first (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this engine runs on one GPU; --gpus {args.gpus} is not supported", file=sys.stderr)
@@ -54,6 +72,8 @@ def main():
 
     from dacti_amd import scene, workloads
     from dacti_amd.integrator import Integrator
+    if args.copy_probe:
+        copy_probe()
 
     t_build = time.perf_counter()
     case = workloads.periodic_box(level=args.level)

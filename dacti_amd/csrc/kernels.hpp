@@ -56,6 +56,16 @@ struct Status {
 
 __device__ __forceinline__ bool is_leaf(const Dev &d, int32_t i) { return d.flags[i] & F_LEAF; }
 
+// Workgroups are dispatched round-robin over the 8 XCDs (each with its own
+// L2).  Give XCD x a contiguous run of logical blocks, so a run of
+// consecutive list entries (spatial neighbours in Morton order) shares one
+// L2.  A bijection of [0, nb).
+__device__ __forceinline__ int xcd_block(int b, int nb) {
+    const int q = nb >> 3, r = nb & 7;
+    const int x = b & 7, k = b >> 3;
+    return x * q + (x < r ? x : r) + k;
+}
+
 // byte-wide atomic OR through the enclosing aligned 32-bit word
 __device__ __forceinline__ void atomic_or_u8(uint8_t *addr, uint8_t v) {
     const uintptr_t a = reinterpret_cast<uintptr_t>(addr);
@@ -97,42 +107,43 @@ __device__ inline int32_t dom_find(const Dev &d, const MConst &c, double *p, int
 // the queried node (2:1 balance keeps real trees at one), and a deeper
 // subtree sets the device error word instead of recursing further.
 template <int NV, typename Leaf>
-__device__ __forceinline__ void interp_rec3(const Dev &d, int32_t idx, double *out, int nch, Leaf &&leaf) {
-    if (d.flags[idx] & F_LEAF) {
+__device__ __forceinline__ void interp_rec3(const uint8_t *flags, const int32_t *child_base, int32_t *err,
+                                            int32_t idx, double *out, int nch, Leaf &&leaf) {
+    if (flags[idx] & F_LEAF) {
         leaf(idx, out);
         return;
     }
     double f1[NV];
     int n1 = 0;
     for (int e = 0; e < NV; ++e) f1[e] = 0.0;
-    const int32_t c1 = d.child_base[idx];
+    const int32_t c1 = child_base[idx];
     for (int a = 0; a < nch; ++a) {
         const int32_t i1 = c1 + a;
-        if (!(d.flags[i1] & F_ACTIVE)) continue;
+        if (!(flags[i1] & F_ACTIVE)) continue;
         double g1[NV];
-        if (d.flags[i1] & F_LEAF) {
+        if (flags[i1] & F_LEAF) {
             leaf(i1, g1);
         } else {
             double f2[NV];
             int n2 = 0;
             for (int e = 0; e < NV; ++e) f2[e] = 0.0;
-            const int32_t c2 = d.child_base[i1];
+            const int32_t c2 = child_base[i1];
             for (int b = 0; b < nch; ++b) {
                 const int32_t i2 = c2 + b;
-                if (!(d.flags[i2] & F_ACTIVE)) continue;
+                if (!(flags[i2] & F_ACTIVE)) continue;
                 double g2[NV];
-                if (d.flags[i2] & F_LEAF) {
+                if (flags[i2] & F_LEAF) {
                     leaf(i2, g2);
                 } else {
                     double f3[NV];
                     int n3 = 0;
                     for (int e = 0; e < NV; ++e) f3[e] = 0.0;
-                    const int32_t c3 = d.child_base[i2];
+                    const int32_t c3 = child_base[i2];
                     for (int c = 0; c < nch; ++c) {
                         const int32_t i3 = c3 + c;
-                        if (!(d.flags[i3] & F_ACTIVE)) continue;
+                        if (!(flags[i3] & F_ACTIVE)) continue;
                         double g3[NV];
-                        if (!(d.flags[i3] & F_LEAF)) atomicOr(d.err, 1);
+                        if (!(flags[i3] & F_LEAF)) atomicOr(err, 1);
                         leaf(i3, g3);
                         for (int e = 0; e < NV; ++e) f3[e] = f3[e] + g3[e];
                         n3++;
@@ -150,26 +161,37 @@ __device__ __forceinline__ void interp_rec3(const Dev &d, int32_t idx, double *o
     for (int e = 0; e < NV; ++e) out[e] = f1[e] / n1;
 }
 
-template <int MODEL, int D>
-__device__ __noinline__ void interp_p(const Dev &d, int32_t idx, double *out) {
-    constexpr int NE = Sz<MODEL, D>::NE;
-    interp_rec3<NE>(d, idx, out, 1 << D, [&](int32_t i, double *o) {
-        for (int e = 0; e < NE; ++e) o[e] = d.p[e * d.C + i];
+// The non-inlined walks take plain pointers: a reference to the kernel's
+// by-value Dev argument would make every thread copy it to scratch.
+template <int NE, int NCH>
+__device__ __noinline__ void interp_vec(const uint8_t *flags, const int32_t *child_base, int32_t *err,
+                                        const double *field, int64_t C, int32_t idx, double *out) {
+    interp_rec3<NE>(flags, child_base, err, idx, out, NCH, [&](int32_t i, double *o) {
+        for (int e = 0; e < NE; ++e) o[e] = field[e * C + i];
     });
 }
 
-template <int D>
-__device__ __noinline__ double interp_t(const Dev &d, int32_t idx) {
+template <int NCH>
+__device__ __noinline__ double interp_scalar(const uint8_t *flags, const int32_t *child_base, int32_t *err,
+                                             const double *field, int32_t idx) {
     double o[1];
-    interp_rec3<1>(d, idx, o, 1 << D, [&](int32_t i, double *r) { r[0] = d.t[i]; });
+    interp_rec3<1>(flags, child_base, err, idx, o, NCH, [&](int32_t i, double *r) { r[0] = field[i]; });
     return o[0];
 }
 
+template <int MODEL, int D>
+__device__ __forceinline__ void interp_p(const Dev &d, int32_t idx, double *out) {
+    interp_vec<Sz<MODEL, D>::NE, (1 << D)>(d.flags, d.child_base, d.err, d.p, d.C, idx, out);
+}
+
 template <int D>
-__device__ __noinline__ double interp_k(const Dev &d, int32_t idx, int kk) {
-    double o[1];
-    interp_rec3<1>(d, idx, o, 1 << D, [&](int32_t i, double *r) { r[0] = d.k[kk * d.C + i]; });
-    return o[0];
+__device__ __forceinline__ double interp_t(const Dev &d, int32_t idx) {
+    return interp_scalar<(1 << D)>(d.flags, d.child_base, d.err, d.t, idx);
+}
+
+template <int D>
+__device__ __forceinline__ double interp_k(const Dev &d, int32_t idx, int kk) {
+    return interp_scalar<(1 << D)>(d.flags, d.child_base, d.err, d.k + kk * d.C, idx);
 }
 
 // Active leaves of subtree `root` touching its face (dim, side), visited in
@@ -325,7 +347,7 @@ __device__ inline void model_flux(const MConst &c, const Side<MODEL, D> &n, doub
 // Returns F with the reference's sign convention (x receives -F if dir==1,
 // +F if dir==0; the neighbour the opposite).
 template <int MODEL, int D>
-__device__ inline void face_flux_sides(const MConst &c, const Side<MODEL, D> &sx, double xc, double xs, int xlev,
+__device__ __noinline__ void face_flux_sides(const MConst &c, const Side<MODEL, D> &sx, double xc, double xs, int xlev,
                                        const Side<MODEL, D> &sn, double nbc, int dim, int dir, double dt,
                                        double *F) {
     const double fc = xc + xs * (dir - 0.5);
@@ -345,8 +367,8 @@ __device__ inline void face_flux_sides(const MConst &c, const Side<MODEL, D> &sx
 
 // Same, loading x's side from memory; sn is x's neighbour across the face.
 template <int MODEL, int D>
-__device__ inline void owner_face_flux_with(const Dev &d, const MConst &c, int32_t x, const Side<MODEL, D> &sn,
-                                            int32_t nb, int dim, int dir, double dt, double *F) {
+__device__ __noinline__ void owner_face_flux_with(const Dev &d, const MConst &c, int32_t x, const Side<MODEL, D> &sn,
+                                                  int32_t nb, int dim, int dir, double dt, double *F) {
     const int64_t C = d.C;
     Side<MODEL, D> sx;
     load_side<MODEL, D>(d, x, dim, sx);
@@ -552,7 +574,7 @@ __global__ void __launch_bounds__(256) k_flux_simple(Dev d, MConst c, const int3
                                                       int L, double dt, int32_t *rest, int32_t *n_rest) {
     using S = Sz<MODEL, D>;
     constexpr int NF = S::NF, NE = S::NE;
-    const int li = blockIdx.x * blockDim.x + threadIdx.x;
+    const int li = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (li >= nl) return;
     const int64_t C = d.C;
     const int32_t y = list[li];
@@ -665,7 +687,7 @@ __global__ void __launch_bounds__(256) k_update(Dev d, MConst c, const int32_t *
                                                  int signal, Status *st) {
     using S = Sz<MODEL, D>;
     constexpr int NE = S::NE, NF = S::NF, IU = S::IU, IP = S::IP;
-    const int li = blockIdx.x * blockDim.x + threadIdx.x;
+    const int li = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (li >= nl) return;
     const int64_t C = d.C;
     const int32_t y = list[li];
@@ -836,12 +858,65 @@ __device__ inline void lts_decide(const Dev &d, const MConst &c, int32_t y, cons
     d.nextc[y] = nc;
 }
 
+// One neighbour of a cell for the sweep (slopes.hpp:21-55 and
+// adaptive_refinement.hpp:38-64): pv is the value the slope uses, qv the
+// value the error estimator uses, h false for a boundary face.  DIR is a
+// template parameter so the per-side arrays stay in registers.
+template <int MODEL, int D, int DIR>
+__device__ __forceinline__ void sweep_side(const Dev &d, const MConst &c, int32_t y, int dim, const double *pc,
+                                           const double *ctr, double ys, double yt, int ylev, int L, int pass,
+                                           double *pv, double *qv, double &tv, double &xv, bool &hv,
+                                           bool &deferred) {
+    using S = Sz<MODEL, D>;
+    constexpr int NE = S::NE;
+    const int64_t C = d.C;
+    const int32_t nb = d.nbr[(2 * dim + DIR) * C + y];
+    hv = true;
+    if (nb == 0) {
+        boundary_value<MODEL, D>(c, bc_face(d, y, 2 * dim + DIR), pc, yt, pv);
+        xv = DIR ? ys : -ys;
+        tv = yt;
+        hv = false;
+        return;
+    }
+    if (!(d.flags[nb] & F_LEAF)) {
+        interp_p<MODEL, D>(d, nb, pv);
+        tv = interp_t<D>(d, nb);
+        for (int e = 0; e < NE; ++e) qv[e] = pv[e];
+    } else {
+        for (int e = 0; e < NE; ++e) pv[e] = d.p[e * C + nb];
+        tv = d.t[nb];
+        if (d.level[nb] < ylev) {
+            if (pass == 0 && d.cluster[nb] == L) {
+                deferred = true;
+            } else {
+                // the coarser leaf's linear reconstruction at the neighbour
+                // position (adaptive_refinement.hpp:48-59)
+                double x[D];
+                for (int a = 0; a < D; ++a) x[a] = ctr[a];
+                x[dim] = ctr[dim] + ys * (2 * DIR - 1.0);
+                if (c.periodic[dim] && DIR == 0 && x[dim] < -0.5 * c.active_size[dim]) x[dim] += c.active_size[dim];
+                if (c.periodic[dim] && DIR == 1 && x[dim] > 0.5 * c.active_size[dim]) x[dim] -= c.active_size[dim];
+                for (int e = 0; e < NE; ++e) qv[e] = pv[e];
+                for (int a = 0; a < D; ++a) {
+                    const double dx = x[a] - d.center[a * C + nb];
+                    for (int e = 0; e < NE; ++e) qv[e] = qv[e] + d.slope[(a * NE + e) * C + nb] * dx;
+                }
+            }
+        } else {
+            for (int e = 0; e < NE; ++e) qv[e] = pv[e];
+        }
+    }
+    const double ns = d.level[nb] == ylev ? ys : d.size[nb];
+    xv = DIR ? 0.5 * (ys + ns) : -0.5 * (ys + ns);
+}
+
 template <int MODEL, int D>
 __global__ void __launch_bounds__(256) k_sweep(Dev d, MConst c, const int32_t *__restrict__ list, int nl, int L,
                                                 double dt, int counter, int pass, Status *st) {
     using S = Sz<MODEL, D>;
     constexpr int NE = S::NE;
-    const int li = blockIdx.x * blockDim.x + threadIdx.x;
+    const int li = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (li >= nl) return;
     const int64_t C = d.C;
     const int32_t y = list[li];
@@ -865,60 +940,8 @@ __global__ void __launch_bounds__(256) k_sweep(Dev d, MConst c, const int32_t *_
         double pn[NE], pp[NE], qn[NE], qp[NE], xn, xp, tn, tp;
         bool hn, hp;
         const double xc = 0.0;
-        for (int dir = 0; dir < 2; ++dir) {
-            const int32_t nb = d.nbr[(2 * dim + dir) * C + y];
-            double *pv = dir ? pp : pn;
-            double *qv = dir ? qp : qn;
-            double tv, xv;
-            bool hv = true;
-            if (nb == 0) {
-                boundary_value<MODEL, D>(c, bc_face(d, y, 2 * dim + dir), pc, yt, pv);
-                xv = dir ? ys : -ys;
-                tv = yt;
-                hv = false;
-            } else {
-                if (!(d.flags[nb] & F_LEAF)) {
-                    interp_p<MODEL, D>(d, nb, pv);
-                    tv = interp_t<D>(d, nb);
-                    for (int e = 0; e < NE; ++e) qv[e] = pv[e];
-                } else {
-                    for (int e = 0; e < NE; ++e) pv[e] = d.p[e * C + nb];
-                    tv = d.t[nb];
-                    if (d.level[nb] < ylev) {
-                        if (pass == 0 && d.cluster[nb] == L) {
-                            deferred = true;
-                        } else {
-                            // the coarser leaf's linear reconstruction at the
-                            // neighbour position (adaptive_refinement.hpp:48-59)
-                            double x[D];
-                            for (int a = 0; a < D; ++a) x[a] = ctr[a];
-                            x[dim] = ctr[dim] + ys * (2 * dir - 1.0);
-                            if (c.periodic[dim] && dir == 0 && x[dim] < -0.5 * c.active_size[dim])
-                                x[dim] += c.active_size[dim];
-                            if (c.periodic[dim] && dir == 1 && x[dim] > 0.5 * c.active_size[dim])
-                                x[dim] -= c.active_size[dim];
-                            for (int e = 0; e < NE; ++e) qv[e] = pv[e];
-                            for (int a = 0; a < D; ++a) {
-                                const double dx = x[a] - d.center[a * C + nb];
-                                for (int e = 0; e < NE; ++e) qv[e] = qv[e] + d.slope[(a * NE + e) * C + nb] * dx;
-                            }
-                        }
-                    } else {
-                        for (int e = 0; e < NE; ++e) qv[e] = pv[e];
-                    }
-                }
-                xv = dir ? 0.5 * (ys + d.size[nb]) : -0.5 * (ys + d.size[nb]);
-            }
-            if (dir) {
-                xp = xv;
-                tp = tv;
-                hp = hv;
-            } else {
-                xn = xv;
-                tn = tv;
-                hn = hv;
-            }
-        }
+        sweep_side<MODEL, D, 0>(d, c, y, dim, pc, ctr, ys, yt, ylev, L, pass, pn, qn, tn, xn, hn, deferred);
+        sweep_side<MODEL, D, 1>(d, c, y, dim, pc, ctr, ys, yt, ylev, L, pass, pp, qp, tp, xp, hp, deferred);
         double sl[NE];
         for (int e = 0; e < NE; ++e) {
             const double sn = (pc[e] - pn[e]) / (xc - xn);
@@ -979,7 +1002,7 @@ __global__ void __launch_bounds__(256) k_sweep(Dev d, MConst c, const int32_t *_
 
 // apply the stored LTS decisions (impl/adaptive_refinement.hpp:308-321) and
 // clear the marks of the listed cells
-__global__ void k_apply(Dev d, const int32_t *__restrict__ list, int nl, int L) {
+static __global__ void k_apply(Dev d, const int32_t *__restrict__ list, int nl, int L) {
     const int li = blockIdx.x * blockDim.x + threadIdx.x;
     if (li >= nl) return;
     const int32_t y = list[li];
@@ -1148,72 +1171,6 @@ __global__ void k_smooth_curvature(Dev d, MConst c, const int32_t *__restrict__ 
 }
 
 // --------------------------------------------------- refinement flagging
-// error estimate + split / unsplit marks: impl/adaptive_refinement.hpp:18-94,
-// weighted_laplacian::estimate error_estimators.hpp:19-37
-template <int MODEL, int D>
-__global__ void __launch_bounds__(256) k_flag(Dev d, MConst c, const int32_t *__restrict__ list, int nl,
-                                               int counter, Status *st) {
-    using S = Sz<MODEL, D>;
-    constexpr int NE = S::NE;
-    const int li = blockIdx.x * blockDim.x + threadIdx.x;
-    if (li >= nl) return;
-    const int64_t C = d.C;
-    const int32_t y = list[li];
-    double pc[NE], ctr[D];
-    for (int e = 0; e < NE; ++e) pc[e] = d.p[e * C + y];
-    for (int i = 0; i < D; ++i) ctr[i] = d.center[i * C + y];
-    const double ys = d.size[y];
-    const int ylev = d.level[y];
-    double err = 0.0;
-    for (int dim = 0; dim < D; ++dim) {
-        double d2[NE];
-        for (int e = 0; e < NE; ++e) d2[e] = pc[e] * -2.0;
-        for (int dir = 0; dir < 2; ++dir) {
-            const int32_t nb = d.nbr[(2 * dim + dir) * C + y];
-            if (nb == 0) {
-                for (int e = 0; e < NE; ++e) d2[e] = 0.0;
-                break;
-            }
-            double q[NE];
-            if (!(d.flags[nb] & F_LEAF)) {
-                interp_p<MODEL, D>(d, nb, q);
-            } else if (d.level[nb] < ylev) {
-                double x[D];
-                for (int i = 0; i < D; ++i) x[i] = ctr[i];
-                x[dim] = ctr[dim] + ys * (2 * dir - 1.0);
-                if (c.periodic[dim] && dir == 0 && x[dim] < -0.5 * c.active_size[dim]) x[dim] += c.active_size[dim];
-                if (c.periodic[dim] && dir == 1 && x[dim] > 0.5 * c.active_size[dim]) x[dim] -= c.active_size[dim];
-                for (int e = 0; e < NE; ++e) q[e] = d.p[e * C + nb];
-                for (int a = 0; a < D; ++a) {
-                    const double dx = x[a] - d.center[a * C + nb];
-                    for (int e = 0; e < NE; ++e) q[e] = q[e] + d.slope[(a * NE + e) * C + nb] * dx;
-                }
-            } else {
-                for (int e = 0; e < NE; ++e) q[e] = d.p[e * C + nb];
-            }
-            for (int e = 0; e < NE; ++e) d2[e] = d2[e] + q[e];
-        }
-        double sq = 0.0;
-        for (int e = 0; e < NE; ++e) {
-            const double v = d2[e] * c.weights[e];
-            sq += v * v;
-        }
-        err += sq;
-    }
-    err = sqrt(err);
-    d.error[y] = err;
-    uint8_t fl = d.flags[y];
-    if (fl & F_CAN_SPLIT) {
-        if (!scene_can_split<D>(c, ctr, ylev)) fl &= ~F_CAN_SPLIT;
-        d.flags[y] = fl;
-    }
-    uint8_t m = 0;
-    if ((err > c.max_error && (fl & F_CAN_SPLIT)) || (fl & F_SHOULD_SPLIT)) m |= MK_SPLIT;
-    if (err < c.min_error && (fl & F_CAN_UNSPLIT) && ylev > 0 && counter == 1) m |= MK_UNSPLIT_CAND;
-    d.mark[y] = m;
-    if (m) atomicAdd(&st->n_marked, 1);
-}
-
 // unsplit detection (:115-149) and split readiness (:154-165)
 template <int D>
 __global__ void k_mark2(Dev d, const int32_t *__restrict__ list, int nl, int L, uint8_t *ready_new,
@@ -1435,7 +1392,7 @@ __device__ void interpolated_cell_acc(const Dev &d, int32_t idx, double &t, doub
     for (int e = 0; e < S::NGX; ++e) ts[e] = ts[e] + cts[e];
 }
 
-__global__ void k_clear_marks(Dev d, const int32_t *__restrict__ list, int nl) {
+static __global__ void k_clear_marks(Dev d, const int32_t *__restrict__ list, int nl) {
     const int li = blockIdx.x * blockDim.x + threadIdx.x;
     if (li >= nl) return;
     d.mark[list[li]] = 0;
@@ -1507,7 +1464,7 @@ __global__ void __launch_bounds__(256) k_reassign(Dev d, MConst c, const int32_t
 }
 
 // flush_acti_buffer: impl/acti_utils.hpp:43-50
-__global__ void k_flush(Dev d, int64_t n, int L) {
+static __global__ void k_flush(Dev d, int64_t n, int L) {
     const int64_t i = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
     if (i < 1 || i >= n) return;
     if (d.buffer[i] == L) {
@@ -1544,7 +1501,7 @@ __global__ void k_init_cells(Dev d, MConst c, int64_t n) {
 }
 
 // assign_to_acti_clusters: impl/acti_utils.hpp:10-24
-__global__ void k_assign(Dev d, int64_t n, double global_dt, double max_cfl) {
+static __global__ void k_assign(Dev d, int64_t n, double global_dt, double max_cfl) {
     const int64_t i = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
     if (i < 1 || i >= n) return;
     const uint8_t fl = d.flags[i];
@@ -1557,7 +1514,7 @@ __global__ void k_assign(Dev d, int64_t n, double global_dt, double max_cfl) {
     d.cluster[i] = (int8_t) lv;
 }
 
-__global__ void k_level_hist(const int8_t *__restrict__ cluster, int64_t n, unsigned long long *hist) {
+static __global__ void k_level_hist(const int8_t *__restrict__ cluster, int64_t n, unsigned long long *hist) {
     __shared__ unsigned int h[32];
     if (threadIdx.x < 32) h[threadIdx.x] = 0;
     __syncthreads();
