@@ -149,3 +149,35 @@ def test_loading_requires_built_library(monkeypatch, tmp_path):
     monkeypatch.setattr(_lib, "_lib", None)
     with pytest.raises(RuntimeError):
         _lib.load()
+
+
+CPP_PROBE = r"""
+#include "dacti/integrator.hpp"
+#include <cstdio>
+int main() {
+    auto s = dacti::model_sizes(DACTI_MODEL_MULTI, 3);
+    std::printf("%d %d %d %d %d\n", s.dim, s.num_eqns, s.num_flux, s.num_parm, s.num_grad);
+    try {
+        dacti::model_sizes(7, 3);
+    } catch (const dacti::Error &e) {
+        std::printf("error %d\n", e.code);
+    }
+    return 0;
+}
+"""
+
+
+def test_cpp_facade_compiles_and_links(tmp_path):
+    import shutil
+    import subprocess
+    if shutil.which("g++") is None:
+        pytest.skip("no g++")
+    src = tmp_path / "probe.cpp"
+    src.write_text(CPP_PROBE)
+    exe = tmp_path / "probe"
+    libdir = os.path.dirname(_lib.LIB_PATH)
+    subprocess.run(["g++", "-std=c++17", "-Wall", "-Wextra", "-Werror", "-I", os.path.join(ROOT, "include"),
+                    str(src), "-L", libdir, "-ldacti_gpu", f"-Wl,-rpath,{libdir}", "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split("\n")
+    assert out[0] == "3 7 11 7 6"
+    assert out[1] == "error -1"
