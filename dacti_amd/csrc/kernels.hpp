@@ -52,6 +52,7 @@ struct Status {
     int32_t lowered;
     int32_t dropped;
     int32_t raised[16];
+    int32_t n_list;         // size of the substep's cluster list as selected on the device
 };
 
 __device__ __forceinline__ bool is_leaf(const Dev &d, int32_t i) { return d.flags[i] & F_LEAF; }
@@ -921,6 +922,7 @@ __global__ void __launch_bounds__(256) k_sweep(Dev d, MConst c, const int32_t *_
     const int64_t C = d.C;
     const int32_t y = list[li];
     if (pass == 1 && !(d.mark[y] & MK_DEFER)) return;
+    if (st->nan_flag) return;  // integrate() stops after the update (impl/integrate.hpp:50)
     double pc[NE], ctr[D], rhs[NE];
     for (int e = 0; e < NE; ++e) pc[e] = d.p[e * C + y];
     for (int a = 0; a < D; ++a) ctr[a] = d.center[a * C + y];

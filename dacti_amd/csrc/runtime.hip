@@ -124,6 +124,29 @@ int build_list(dacti_handle *h, int L, int64_t *nl, int32_t *out) {
     return DACTI_OK;
 }
 
+// cluster-L list without waiting for its size (the host knows it from the
+// level histogram); the device count goes to *count
+int build_list_async(dacti_handle *h, int L, int32_t *out, int32_t *count) {
+    hipcub::CountingInputIterator<int32_t> it(1);
+    size_t bytes = h->tmp_bytes;
+    HIPCHK(h, hipcub::DeviceSelect::If(h->d_tmp, bytes, it, out, count, (int) (h->n - 1), ClusterIs{h->d.cluster, L},
+                                       h->stream));
+    return DACTI_OK;
+}
+
+// level histogram and status in one round trip
+int finish_substep(dacti_handle *h) {
+    HIPCHK(h, hipMemsetAsync(h->d_hist, 0, sizeof(unsigned long long) * 16, h->stream));
+    const int64_t nb = std::min<int64_t>(blocks(h->n, 256), 4096);
+    k_level_hist<<<(unsigned) std::max<int64_t>(nb, 1), 256, 0, h->stream>>>(h->d.cluster, h->n, h->d_hist);
+    HIPCHK(h, hipGetLastError());
+    HIPCHK(h, hipMemcpyAsync(h->h_hist, h->d_hist, sizeof(unsigned long long) * 16, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipMemcpyAsync(h->h_status, h->d_status, sizeof(Status), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    for (int l = 0; l < 16; ++l) h->counts[l] = (int64_t) h->h_hist[l];
+    return DACTI_OK;
+}
+
 int read_status(dacti_handle *h) {
     HIPCHK(h, hipMemcpyAsync(h->h_status, h->d_status, sizeof(Status), hipMemcpyDeviceToHost, h->stream));
     HIPCHK(h, hipStreamSynchronize(h->stream));

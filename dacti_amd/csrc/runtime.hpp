@@ -109,8 +109,10 @@ struct ClusterIs {
 int alloc_pool(dacti_handle *h, int64_t C, Dev *old, int64_t n);
 int ensure_capacity(dacti_handle *h, int64_t need);
 int build_list(dacti_handle *h, int L, int64_t *nl, int32_t *out = nullptr);
+int build_list_async(dacti_handle *h, int L, int32_t *out, int32_t *count);
 int read_status(dacti_handle *h);
 int refresh_counts(dacti_handle *h);
+int finish_substep(dacti_handle *h);
 int gc(dacti_handle *h);
 
 // per-model entry points, one translation unit each (variant_*.hip)
@@ -193,12 +195,14 @@ struct Ops {
     // one level substep: impl/integrate.hpp:42-64; returns 1 on force_break
     static int substep(dacti_handle *h, int L, int signal, int *force_break) {
         const double dt = h->global_dt / (1 << L);
-        int64_t nl64 = 0;
-        int rc = build_list(h, L, &nl64);
-        if (rc) return rc;
-        const int nl = (int) nl64;
-        HIPCHK(h, hipMemsetAsync(h->d_status, 0, sizeof(Status), h->stream));
+        // the cluster size is known from the level histogram of the last substep
+        const int nl = (int) h->counts[L];
+        int rc = 0;
+        bool changed = false;
         if (nl > 0) {
+            HIPCHK(h, hipMemsetAsync(h->d_status, 0, sizeof(Status), h->stream));
+            rc = build_list_async(h, L, h->d_list, &h->d_status->n_list);
+            if (rc) return rc;
             if (h->timing) HIPCHK(h, hipEventRecord(h->ev[0], h->stream));
             k_flux_simple<MODEL, D><<<blocks(nl), 256, 0, h->stream>>>(h->d, h->mc, h->d_list, nl, L, dt, h->d_list2,
                                                                        &h->d_status->n_rest);
@@ -209,13 +213,17 @@ struct Ops {
             if (h->timing) HIPCHK(h, hipEventRecord(h->ev[1], h->stream));
             h->flux_launches++;
             h->cell_updates += nl;
-            rc = read_status(h);
-            if (rc) return rc;
-            if (h->h_status->nan_flag) {
-                *force_break = 1;
-                return 1;
+            if constexpr (MODEL == MULTI) {
+                // the interface chain is not guarded by the NaN flag: stop here
+                rc = read_status(h);
+                if (rc) return rc;
+                if (h->h_status->nan_flag) {
+                    *force_break = 1;
+                    return 1;
+                }
             }
-            // slopes, refinement marks and LTS decisions in one sweep
+            // slopes, refinement marks and LTS decisions in one sweep (a
+            // no-op when the update produced a NaN state)
             k_sweep<MODEL, D><<<blocks(nl), 256, 0, h->stream>>>(h->d, h->mc, h->d_list, nl, L, dt,
                                                                  h->acti_counter[L], 0, h->d_status);
             HIPCHK(h, hipGetLastError());
@@ -223,6 +231,14 @@ struct Ops {
             if (rc) return rc;
             rc = read_status(h);
             if (rc) return rc;
+            if (h->h_status->nan_flag) {
+                *force_break = 1;
+                return 1;
+            }
+            if (h->h_status->n_list != nl) {
+                h->err = "internal: cluster list size differs from the level histogram";
+                return DACTI_E_STATE;
+            }
             if (h->h_status->n_deferred > 0) {
                 k_sweep<MODEL, D><<<blocks(nl), 256, 0, h->stream>>>(h->d, h->mc, h->d_list, nl, L, dt,
                                                                      h->acti_counter[L], 1, h->d_status);
@@ -236,11 +252,23 @@ struct Ops {
             }
             k_apply<<<blocks(nl), 256, 0, h->stream>>>(h->d, h->d_list, nl, L);
             HIPCHK(h, hipGetLastError());
-            if (h->timing) {
-                HIPCHK(h, hipEventRecord(h->ev[2], h->stream));
-            }
-            rc = read_status(h);
+            if (h->timing) HIPCHK(h, hipEventRecord(h->ev[2], h->stream));
+            changed = true;
+        }
+        h->acti_counter[L] = (h->acti_counter[L] + 1) % 2;
+        h->acti_time[L] += h->global_dt / (1 << L);
+        if (h->pending[L] > 0) {
+            k_flush<<<blocks(h->n), 256, 0, h->stream>>>(h->d, h->n, L);
+            HIPCHK(h, hipGetLastError());
+            h->pending[L] = 0;
+            changed = true;
+        }
+        if (changed) {
+            // one round trip: level histogram, status (dev_error, lowered)
+            rc = finish_substep(h);
             if (rc) return rc;
+        }
+        if (nl > 0) {
             if (h->h_status->dev_error) {
                 h->err = "a neighbour subtree is more than three levels deeper than the cell (2:1 balance broken)";
                 return DACTI_E_STATE;
@@ -254,15 +282,6 @@ struct Ops {
             }
             if (L > 0) h->pending[L - 1] += h->h_status->lowered;
         }
-        h->acti_counter[L] = (h->acti_counter[L] + 1) % 2;
-        h->acti_time[L] += h->global_dt / (1 << L);
-        if (h->pending[L] > 0) {
-            k_flush<<<blocks(h->n), 256, 0, h->stream>>>(h->d, h->n, L);
-            HIPCHK(h, hipGetLastError());
-            h->pending[L] = 0;
-        }
-        rc = refresh_counts(h);
-        if (rc) return rc;
         h->n_substeps++;
         return 0;
     }
