@@ -17,6 +17,8 @@ import os
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
@@ -57,6 +59,89 @@ def copy_probe(gib=1.0):
     print(f"[bench] copy probe: {n} bytes device to device", file=sys.stderr, flush=True)
 
 
+def batched(args):
+    """64 independent 2D shock-bubble instances (five-equation model, seeded
+    Mach, radius, density ratio and refinement depth), stepped one integrator
+    at a time ("plain") and as one batch (one launch per op and substep for
+    all instances, autotuned launch, graph replay).  Both start from the same
+    states and time the same global steps, so they do the same cell updates."""
+    from dacti_amd import scene, workloads
+    from dacti_amd.integrator import Batch, Integrator
+    fam = workloads.shock_bubble_family(args.instances, seed=args.seed)
+    trees = [scene.build_tree(c) for c, _ in fam]
+    print(f"[bench] {len(trees)} instance trees, {sum(len(t.leaves()) for t in trees)} leaves", file=sys.stderr,
+          flush=True)
+    warm = max(args.warmup, 6)
+
+    # an instance whose state turns NaN stops (force_break), in both forms
+    plain = [Integrator(c, t) for (c, _), t in zip(fam, trees)]
+    alive = [True] * len(plain)
+
+    def plain_step():
+        for k, g in enumerate(plain):
+            if alive[k] and g.integrate()[1]:
+                alive[k] = False
+
+    tw = time.perf_counter()
+    for w in range(warm):
+        plain_step()
+        print(f"[bench] plain warmup step {w} done at {time.perf_counter() - tw:.1f} s", file=sys.stderr, flush=True)
+    u0 = sum(g.stats().cell_updates for g in plain)
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        plain_step()
+        print(f"[bench] plain step {k} done", file=sys.stderr, flush=True)
+    el_plain = time.perf_counter() - t0
+    up_plain = sum(g.stats().cell_updates for g in plain) - u0
+    print(f"[bench] plain: {args.steps} steps in {el_plain:.3f} s", file=sys.stderr, flush=True)
+
+    members = [Integrator(c, t) for (c, _), t in zip(fam, trees)]
+    b = Batch(members)
+    b.autotune()
+    for w in range(warm):
+        b.integrate()
+        print(f"[bench] batched warmup/tuning step {w} done", file=sys.stderr, flush=True)
+    tuned = b.tune_results()
+    s0 = b.stats()
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        b.integrate()
+        print(f"[bench] batched step {k} done", file=sys.stderr, flush=True)
+    el = time.perf_counter() - t0
+    s1 = b.stats()
+    up = s1.cell_updates - s0.cell_updates
+    print(f"[bench] batched: {args.steps} steps in {el:.3f} s", file=sys.stderr, flush=True)
+    same = all(np.array_equal(a.get_nodes()["u"], m.get_nodes()["u"], equal_nan=True)
+               for a, m in zip(plain, members))
+    blk, grd, gr = b.launch()
+    out = {
+        "metric": "cell-updates/s (HLLC+MUSCL FP64) & %HBM roofline; learned/batched vs plain",
+        "value": up / el,
+        "unit": "cell-updates/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": warm,
+        "ms_per_step": el * 1e3 / args.steps,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"{len(fam)} independent 2D shock-bubble instances (five-equation model), "
+                               "seeded Mach U[1.1,3], radius U[0.1,0.3], density ratio log-U[0.1,10], "
+                               "4-7 levels above a 64x32 base, AMR and LTS on",
+                   "instances": len(fam), "leaves_initial": sum(len(t.leaves()) for t in trees),
+                   "cell_updates_timed": up, "launch": {"block": blk, "grid": grd, "graphs": gr},
+                   "autotune": [{"block": r[0], "grid": r[1], "graphs": r[2], "ms": r[3]} for r in tuned]},
+        "plain": {"value": up_plain / el_plain, "cell_updates_timed": up_plain,
+                  "ms_per_step": el_plain * 1e3 / args.steps},
+        "batched_over_plain": (up / el) / (up_plain / el_plain),
+        "same_state_as_plain": bool(same),
+        "instances_stopped_nan": int(len(plain) - sum(alive)),
+        "roofline": None,
+        "cpu_baseline": None,
+    }
+    b.close()
+    for g in plain + members:
+        g.close()
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -65,10 +150,17 @@ def main():
     ap.add_argument("--level", type=int, default=8)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-probe", action="store_true", help="run a 1 GiB device copy first (counter calibration)")
+    ap.add_argument("--workload", choices=("box", "batched"), default="box",
+                    help="box: 3D periodic Euler box (default); batched: shock-bubble instances, batch vs plain")
+    ap.add_argument("--instances", type=int, default=64)
+    ap.add_argument("--seed", type=int, default=0)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this engine runs on one GPU; --gpus {args.gpus} is not supported", file=sys.stderr)
         sys.exit(2)
+    if args.workload == "batched":
+        batched(args)
+        return
 
     from dacti_amd import scene, workloads
     from dacti_amd.integrator import Integrator

@@ -83,6 +83,19 @@ SIGNATURES = {
     "dacti_set_timing": (C.c_int, [C.c_void_p, C.c_int32]),
     "dacti_destroy": (C.c_int, [C.c_void_p]),
     "dacti_last_error": (C.c_char_p, [C.c_void_p]),
+    "dacti_batch_create": (C.c_int, [C.POINTER(C.c_void_p), C.c_int32, C.POINTER(C.c_void_p)]),
+    "dacti_batch_integrate": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_int32)]),
+    "dacti_batch_run": (C.c_int, [C.c_void_p, C.c_int64, C.c_int32, C.POINTER(C.c_int64)]),
+    "dacti_batch_set_launch": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32]),
+    "dacti_batch_get_launch": (C.c_int, [C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
+                                         C.POINTER(C.c_int32)]),
+    "dacti_batch_autotune": (C.c_int, [C.c_void_p]),
+    "dacti_batch_tune_results": (C.c_int, [C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
+                                           C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.POINTER(C.c_double)]),
+    "dacti_batch_get_stats": (C.c_int, [C.c_void_p, C.POINTER(StatsStruct)]),
+    "dacti_batch_set_timing": (C.c_int, [C.c_void_p, C.c_int32]),
+    "dacti_batch_destroy": (C.c_int, [C.c_void_p]),
+    "dacti_batch_last_error": (C.c_char_p, [C.c_void_p]),
 }
 
 _lib = None

@@ -103,63 +103,68 @@ __device__ inline int32_t dom_find(const Dev &d, const MConst &c, double *p, int
     return idx;
 }
 
-// Domain::interpolate_p / _t / _k: integrator/domain.hpp:84-148.  The
-// reference recurses; here the recursion is unrolled to three levels below
-// the queried node (2:1 balance keeps real trees at one), and a deeper
-// subtree sets the device error word instead of recursing further.
+// Domain::interpolate_p / _t / _k: integrator/domain.hpp:84-148: the mean
+// over a node's active children, each child contributing its leaf value or,
+// recursively, its own children's mean.  The 2:1 balance bounds a subtree's
+// depth only next to the face it shares with a leaf; on its far side the
+// subtree can reach the finest level.  The walk keeps an explicit stack of
+// up to 16 levels in private memory (a non-inlined, rarely taken path) and
+// adds the child means in the reference's order.
 template <int NV, typename Leaf>
-__device__ __forceinline__ void interp_rec3(const uint8_t *flags, const int32_t *child_base, int32_t *err,
+__device__ __forceinline__ void interp_walk(const uint8_t *flags, const int32_t *child_base, int32_t *err,
                                             int32_t idx, double *out, int nch, Leaf &&leaf) {
     if (flags[idx] & F_LEAF) {
         leaf(idx, out);
         return;
     }
-    double f1[NV];
-    int n1 = 0;
-    for (int e = 0; e < NV; ++e) f1[e] = 0.0;
-    const int32_t c1 = child_base[idx];
-    for (int a = 0; a < nch; ++a) {
-        const int32_t i1 = c1 + a;
-        if (!(flags[i1] & F_ACTIVE)) continue;
-        double g1[NV];
-        if (flags[i1] & F_LEAF) {
-            leaf(i1, g1);
-        } else {
-            double f2[NV];
-            int n2 = 0;
-            for (int e = 0; e < NV; ++e) f2[e] = 0.0;
-            const int32_t c2 = child_base[i1];
-            for (int b = 0; b < nch; ++b) {
-                const int32_t i2 = c2 + b;
-                if (!(flags[i2] & F_ACTIVE)) continue;
-                double g2[NV];
-                if (flags[i2] & F_LEAF) {
-                    leaf(i2, g2);
-                } else {
-                    double f3[NV];
-                    int n3 = 0;
-                    for (int e = 0; e < NV; ++e) f3[e] = 0.0;
-                    const int32_t c3 = child_base[i2];
-                    for (int c = 0; c < nch; ++c) {
-                        const int32_t i3 = c3 + c;
-                        if (!(flags[i3] & F_ACTIVE)) continue;
-                        double g3[NV];
-                        if (!(flags[i3] & F_LEAF)) atomicOr(err, 1);
-                        leaf(i3, g3);
-                        for (int e = 0; e < NV; ++e) f3[e] = f3[e] + g3[e];
-                        n3++;
-                    }
-                    for (int e = 0; e < NV; ++e) g2[e] = f3[e] / n3;
-                }
-                for (int e = 0; e < NV; ++e) f2[e] = f2[e] + g2[e];
-                n2++;
+    constexpr int MAXD = 16;
+    double acc[MAXD][NV];
+    int cnt[MAXD], pos[MAXD];
+    int32_t base[MAXD];
+    int dp = 0;
+    for (int e = 0; e < NV; ++e) acc[0][e] = 0.0;
+    cnt[0] = 0;
+    pos[0] = 0;
+    base[0] = child_base[idx];
+    while (true) {
+        if (pos[dp] == nch) {
+            double m[NV];
+            for (int e = 0; e < NV; ++e) m[e] = acc[dp][e] / cnt[dp];
+            if (dp == 0) {
+                for (int e = 0; e < NV; ++e) out[e] = m[e];
+                return;
             }
-            for (int e = 0; e < NV; ++e) g1[e] = f2[e] / n2;
+            --dp;
+            for (int e = 0; e < NV; ++e) acc[dp][e] = acc[dp][e] + m[e];
+            cnt[dp]++;
+            pos[dp]++;
+            continue;
         }
-        for (int e = 0; e < NV; ++e) f1[e] = f1[e] + g1[e];
-        n1++;
+        const int32_t i = base[dp] + pos[dp];
+        const uint8_t fl = flags[i];
+        if (!(fl & F_ACTIVE)) {
+            pos[dp]++;
+            continue;
+        }
+        if (fl & F_LEAF) {
+            double g[NV];
+            leaf(i, g);
+            for (int e = 0; e < NV; ++e) acc[dp][e] = acc[dp][e] + g[e];
+            cnt[dp]++;
+            pos[dp]++;
+            continue;
+        }
+        if (dp + 1 == MAXD) {  // deeper than any tree of 16 levels
+            atomicOr(err, 1);
+            pos[dp]++;
+            continue;
+        }
+        ++dp;
+        for (int e = 0; e < NV; ++e) acc[dp][e] = 0.0;
+        cnt[dp] = 0;
+        pos[dp] = 0;
+        base[dp] = child_base[i];
     }
-    for (int e = 0; e < NV; ++e) out[e] = f1[e] / n1;
 }
 
 // The non-inlined walks take plain pointers: a reference to the kernel's
@@ -167,7 +172,7 @@ __device__ __forceinline__ void interp_rec3(const uint8_t *flags, const int32_t 
 template <int NE, int NCH>
 __device__ __noinline__ void interp_vec(const uint8_t *flags, const int32_t *child_base, int32_t *err,
                                         const double *field, int64_t C, int32_t idx, double *out) {
-    interp_rec3<NE>(flags, child_base, err, idx, out, NCH, [&](int32_t i, double *o) {
+    interp_walk<NE>(flags, child_base, err, idx, out, NCH, [&](int32_t i, double *o) {
         for (int e = 0; e < NE; ++e) o[e] = field[e * C + i];
     });
 }
@@ -176,7 +181,7 @@ template <int NCH>
 __device__ __noinline__ double interp_scalar(const uint8_t *flags, const int32_t *child_base, int32_t *err,
                                              const double *field, int32_t idx) {
     double o[1];
-    interp_rec3<1>(flags, child_base, err, idx, o, NCH, [&](int32_t i, double *r) { r[0] = field[i]; });
+    interp_walk<1>(flags, child_base, err, idx, o, NCH, [&](int32_t i, double *r) { r[0] = field[i]; });
     return o[0];
 }
 
@@ -571,11 +576,10 @@ __global__ void __launch_bounds__(256) k_flux(Dev d, MConst c, const int32_t *__
 // A cell with any other face is appended to `rest` (order irrelevant:
 // cells are independent) and handled by k_flux.
 template <int MODEL, int D>
-__global__ void __launch_bounds__(256) k_flux_simple(Dev d, MConst c, const int32_t *__restrict__ list, int nl,
+__device__ __forceinline__ void flux_simple_body(const Dev &d, const MConst &c, const int32_t *__restrict__ list, int nl, int li,
                                                       int L, double dt, int32_t *rest, int32_t *n_rest) {
     using S = Sz<MODEL, D>;
     constexpr int NF = S::NF, NE = S::NE;
-    const int li = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (li >= nl) return;
     const int64_t C = d.C;
     const int32_t y = list[li];
@@ -684,11 +688,10 @@ __global__ void __launch_bounds__(256) k_flux_simple(Dev d, MConst c, const int3
 // ---------------------------------------------------------------- update
 // cell_update: impl/cell_update.hpp:9-90, multi::source multi.hpp:186-203
 template <int MODEL, int D>
-__global__ void __launch_bounds__(256) k_update(Dev d, MConst c, const int32_t *__restrict__ list, int nl, double dt,
+__device__ __forceinline__ void update_body(const Dev &d, const MConst &c, const int32_t *__restrict__ list, int nl, int li, double dt,
                                                  int signal, Status *st) {
     using S = Sz<MODEL, D>;
     constexpr int NE = S::NE, NF = S::NF, IU = S::IU, IP = S::IP;
-    const int li = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (li >= nl) return;
     const int64_t C = d.C;
     const int32_t y = list[li];
@@ -913,11 +916,10 @@ __device__ __forceinline__ void sweep_side(const Dev &d, const MConst &c, int32_
 }
 
 template <int MODEL, int D>
-__global__ void __launch_bounds__(256) k_sweep(Dev d, MConst c, const int32_t *__restrict__ list, int nl, int L,
+__device__ __forceinline__ void sweep_body(const Dev &d, const MConst &c, const int32_t *__restrict__ list, int nl, int li, int L,
                                                 double dt, int counter, int pass, Status *st) {
     using S = Sz<MODEL, D>;
     constexpr int NE = S::NE;
-    const int li = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (li >= nl) return;
     const int64_t C = d.C;
     const int32_t y = list[li];
@@ -1004,8 +1006,7 @@ __global__ void __launch_bounds__(256) k_sweep(Dev d, MConst c, const int32_t *_
 
 // apply the stored LTS decisions (impl/adaptive_refinement.hpp:308-321) and
 // clear the marks of the listed cells
-static __global__ void k_apply(Dev d, const int32_t *__restrict__ list, int nl, int L) {
-    const int li = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void apply_body(const Dev &d, const int32_t *__restrict__ list, int nl, int li, int L) {
     if (li >= nl) return;
     const int32_t y = list[li];
     if (d.mark[y] & MK_RA_DONE) {
@@ -1028,9 +1029,8 @@ __device__ inline double bc_alpha(const Dev &d, const MConst &c, int32_t y, int 
 
 // smooth_alpha, one Jacobi sweep: impl/smooth.hpp:9-60
 template <int MODEL, int D>
-__global__ void k_smooth_alpha(Dev d, MConst c, const int32_t *__restrict__ list, int nl) {
+__device__ __forceinline__ void smooth_alpha_body(const Dev &d, const MConst &c, const int32_t *__restrict__ list, int nl, int li) {
     using S = Sz<MODEL, D>;
-    const int li = blockIdx.x * blockDim.x + threadIdx.x;
     if (li >= nl) return;
     const int64_t C = d.C;
     const int32_t y = list[li];
@@ -1050,18 +1050,15 @@ __global__ void k_smooth_alpha(Dev d, MConst c, const int32_t *__restrict__ list
     d.scratch[li] = acc / (2 * D);
 }
 
-template <int KK>
-__global__ void k_store_k(Dev d, const int32_t *__restrict__ list, int nl, int kk) {
-    const int li = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void store_k_body(const Dev &d, const int32_t *__restrict__ list, int nl, int li, int kk) {
     if (li >= nl) return;
     d.k[kk * d.C + list[li]] = d.scratch[li];
 }
 
 // compute_normal: impl/normal_curvature.hpp:6-55
 template <int MODEL, int D>
-__global__ void k_normal(Dev d, MConst c, const int32_t *__restrict__ list, int nl) {
+__device__ __forceinline__ void normal_body(const Dev &d, const MConst &c, const int32_t *__restrict__ list, int nl, int li) {
     using S = Sz<MODEL, D>;
-    const int li = blockIdx.x * blockDim.x + threadIdx.x;
     if (li >= nl) return;
     const int64_t C = d.C;
     const int32_t y = list[li];
@@ -1094,9 +1091,8 @@ __global__ void k_normal(Dev d, MConst c, const int32_t *__restrict__ list, int 
 
 // compute_curvature: impl/normal_curvature.hpp:58-102
 template <int MODEL, int D>
-__global__ void k_curvature(Dev d, MConst c, const int32_t *__restrict__ list, int nl) {
+__device__ __forceinline__ void curvature_body(const Dev &d, const MConst &c, const int32_t *__restrict__ list, int nl, int li) {
     using S = Sz<MODEL, D>;
-    const int li = blockIdx.x * blockDim.x + threadIdx.x;
     if (li >= nl) return;
     const int64_t C = d.C;
     const int32_t y = list[li];
@@ -1126,9 +1122,8 @@ __global__ void k_curvature(Dev d, MConst c, const int32_t *__restrict__ list, i
 
 // smooth_curvature (one iteration): impl/smooth.hpp:63-142
 template <int MODEL, int D>
-__global__ void k_smooth_curvature(Dev d, MConst c, const int32_t *__restrict__ list, int nl) {
+__device__ __forceinline__ void smooth_curvature_body(const Dev &d, const MConst &c, const int32_t *__restrict__ list, int nl, int li) {
     using S = Sz<MODEL, D>;
-    const int li = blockIdx.x * blockDim.x + threadIdx.x;
     if (li >= nl) return;
     const int64_t C = d.C;
     const int32_t y = list[li];
@@ -1466,8 +1461,7 @@ __global__ void __launch_bounds__(256) k_reassign(Dev d, MConst c, const int32_t
 }
 
 // flush_acti_buffer: impl/acti_utils.hpp:43-50
-static __global__ void k_flush(Dev d, int64_t n, int L) {
-    const int64_t i = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void flush_node(const Dev &d, int64_t i, int64_t n, int L) {
     if (i < 1 || i >= n) return;
     if (d.buffer[i] == L) {
         d.buffer[i] = -1;
@@ -1526,6 +1520,269 @@ static __global__ void k_level_hist(const int8_t *__restrict__ cluster, int64_t 
     }
     __syncthreads();
     if (threadIdx.x < 16 && h[threadIdx.x]) atomicAdd(&hist[threadIdx.x], (unsigned long long) h[threadIdx.x]);
+}
+
+// ------------------------------------------------------------ batched launches
+// One launch of an op covers one substep of every instance of a batch (a
+// single integrator is a batch of one).  Item i describes an instance's node
+// pool, its constants (device copy), its cluster list and its dt, counter
+// and flags for this substep.  The list work of item i is cut into tiles of
+// blockDim.x entries, its pool work into tiles of POOL_TILE nodes; toff[i] is
+// the first tile of item i and toff[n_items] the total.  A persistent grid
+// strides over the tiles; with a grid that is a multiple of 8 blocks the
+// tile index is remapped so that each XCD takes a contiguous range.  The
+// instances' constants come through their own restrict-qualified array, so
+// the compiler may keep them in scalar registers across the ops' stores.
+struct Item {
+    Dev d;
+    int32_t *list;
+    int32_t *rest;
+    Status *st;
+    unsigned long long *hist;  // 16 per-level cluster counts
+    int64_t n;                 // pool size
+    double dt;
+    int32_t nl, L, counter, signal, flush, pad;
+};
+constexpr int POOL_TILE = 2048;
+
+__device__ __forceinline__ int find_item(const int *__restrict__ toff, int n_items, int tile) {
+    int lo = 0, hi = n_items - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (toff[mid] <= tile) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+template <class Op>
+__global__ void __launch_bounds__(256) k_batch(const Item *__restrict__ items, const MConst *__restrict__ mcs,
+                                               const int *__restrict__ toff, int n_items, Op op) {
+    const int total = toff[n_items];
+    for (int t = blockIdx.x; t < total; t += gridDim.x) {
+        const int tile = xcd_block(t, total);
+        const int i = find_item(toff, n_items, tile);
+        op(items[i], mcs[i], (tile - toff[i]) * (int) blockDim.x + (int) threadIdx.x);
+    }
+}
+
+template <int MODEL, int D>
+struct OpFluxSimple {
+    __device__ __forceinline__ void operator()(const Item &it, const MConst &c, int li) const {
+        flux_simple_body<MODEL, D>(it.d, c, it.list, it.nl, li, it.L, it.dt, it.rest, &it.st->n_rest);
+    }
+};
+template <int MODEL, int D>
+struct OpFluxRest {
+    __device__ __forceinline__ void operator()(const Item &it, const MConst &c, int li) const {
+        if (li >= it.st->n_rest) return;
+        flux_cell<MODEL, D>(it.d, c, it.rest[li], it.L, it.dt);
+    }
+};
+template <int MODEL, int D>
+struct OpUpdate {
+    __device__ __forceinline__ void operator()(const Item &it, const MConst &c, int li) const {
+        update_body<MODEL, D>(it.d, c, it.list, it.nl, li, it.dt, it.signal, it.st);
+    }
+};
+template <int MODEL, int D>
+struct OpSweep {
+    int pass;
+    __device__ __forceinline__ void operator()(const Item &it, const MConst &c, int li) const {
+        if (pass == 1 && it.st->n_deferred == 0) return;
+        sweep_body<MODEL, D>(it.d, c, it.list, it.nl, li, it.L, it.dt, it.counter, pass, it.st);
+    }
+};
+// the interface chain (impl/integrate.hpp:53-58) is skipped for an instance
+// whose update produced a NaN state
+template <int MODEL, int D>
+struct OpSmoothAlpha {
+    __device__ __forceinline__ void operator()(const Item &it, const MConst &c, int li) const {
+        if (it.st->nan_flag) return;
+        smooth_alpha_body<MODEL, D>(it.d, c, it.list, it.nl, li);
+    }
+};
+struct OpStoreK {
+    int kk;
+    __device__ __forceinline__ void operator()(const Item &it, const MConst &c, int li) const {
+        if (it.st->nan_flag) return;
+        store_k_body(it.d, it.list, it.nl, li, kk);
+    }
+};
+template <int MODEL, int D>
+struct OpNormal {
+    __device__ __forceinline__ void operator()(const Item &it, const MConst &c, int li) const {
+        if (it.st->nan_flag) return;
+        normal_body<MODEL, D>(it.d, c, it.list, it.nl, li);
+    }
+};
+template <int MODEL, int D>
+struct OpCurvature {
+    __device__ __forceinline__ void operator()(const Item &it, const MConst &c, int li) const {
+        if (it.st->nan_flag) return;
+        curvature_body<MODEL, D>(it.d, c, it.list, it.nl, li);
+    }
+};
+template <int MODEL, int D>
+struct OpSmoothCurvature {
+    __device__ __forceinline__ void operator()(const Item &it, const MConst &c, int li) const {
+        if (it.st->nan_flag) return;
+        smooth_curvature_body<MODEL, D>(it.d, c, it.list, it.nl, li);
+    }
+};
+struct OpApply {
+    __device__ __forceinline__ void operator()(const Item &it, const MConst &c, int li) const {
+        apply_body(it.d, it.list, it.nl, li, it.L);
+    }
+};
+
+// exclusive scan of one value per thread over a block of 1024 threads
+__device__ __forceinline__ int block_exclusive_scan_1024(int v, int *lds, int &total) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    int x = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int y = __shfl_up(x, o);
+        if (lane >= o) x += y;
+    }
+    if (lane == 63) lds[w] = x;
+    __syncthreads();
+    if (threadIdx.x < 16) {
+        int a = lds[threadIdx.x];
+        for (int o = 1; o < 16; o <<= 1) {
+            const int y = __shfl_up(a, o, 16);
+            if ((int) threadIdx.x >= o) a += y;
+        }
+        lds[16 + threadIdx.x] = a;
+    }
+    __syncthreads();
+    total = lds[31];
+    const int r = (w ? lds[16 + w - 1] : 0) + x - v;
+    __syncthreads();
+    return r;
+}
+
+// Cluster-L lists of the batch, in node order (the order of the reference's
+// acti_clusters after a rebuild is irrelevant to the results; node order keeps
+// the sweeps coalesced): per pool tile a count, one scan per item, a scatter.
+static __global__ void __launch_bounds__(256) k_sel_count(const Item *__restrict__ items, const int *__restrict__ toff,
+                                                   int n_items, int *tilecnt) {
+    __shared__ int wc[4];
+    const int total = toff[n_items];
+    for (int t = blockIdx.x; t < total; t += gridDim.x) {
+        const int tile = xcd_block(t, total);
+        const int i = find_item(toff, n_items, tile);
+        const Item &it = items[i];
+        int cnt = 0;
+        if (it.nl > 0) {
+            const int64_t base = (int64_t) (tile - toff[i]) * POOL_TILE;
+            for (int k = 0; k < POOL_TILE; k += 256) {
+                const int64_t id = base + k + threadIdx.x;
+                const bool pr = id >= 1 && id < it.n && it.d.cluster[id] == it.L;
+                cnt += __popcll(__ballot(pr));
+            }
+        }
+        if ((threadIdx.x & 63) == 0) wc[threadIdx.x >> 6] = cnt;
+        __syncthreads();
+        if (threadIdx.x == 0) tilecnt[tile] = wc[0] + wc[1] + wc[2] + wc[3];
+        __syncthreads();
+    }
+}
+
+// one block of 1024 threads: tile offsets within each item, the item's list
+// size, and a cleared status
+static __global__ void __launch_bounds__(1024) k_sel_scan(const Item *__restrict__ items, const int *__restrict__ toff,
+                                                   int n_items, int *tilecnt) {
+    __shared__ int lds[32];
+    for (int i = 0; i < n_items; ++i) {
+        const int a = toff[i], b = toff[i + 1];
+        int running = 0;
+        for (int s = a; s < b; s += 1024) {
+            const int idx = s + threadIdx.x;
+            const int v = idx < b ? tilecnt[idx] : 0;
+            int tot = 0;
+            const int ex = block_exclusive_scan_1024(v, lds, tot);
+            if (idx < b) tilecnt[idx] = running + ex;
+            running += tot;
+        }
+        if (threadIdx.x == 0) {
+            Status *st = items[i].st;
+            *st = Status{};
+            st->n_list = running;
+        }
+    }
+}
+
+static __global__ void __launch_bounds__(256) k_sel_scatter(const Item *__restrict__ items, const int *__restrict__ toff,
+                                                     int n_items, const int *__restrict__ tileoff) {
+    __shared__ int wc[4];
+    const int total = toff[n_items];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const uint64_t lt = lane ? (~0ull >> (64 - lane)) : 0ull;
+    for (int t = blockIdx.x; t < total; t += gridDim.x) {
+        const int tile = xcd_block(t, total);
+        const int i = find_item(toff, n_items, tile);
+        const Item &it = items[i];
+        if (it.nl == 0) continue;  // uniform over the block
+        const int64_t base = (int64_t) (tile - toff[i]) * POOL_TILE;
+        int pos = tileoff[tile];
+        for (int k = 0; k < POOL_TILE; k += 256) {
+            const int64_t id = base + k + threadIdx.x;
+            const bool pr = id >= 1 && id < it.n && it.d.cluster[id] == it.L;
+            const uint64_t m = __ballot(pr);
+            if (lane == 0) wc[w] = __popcll(m);
+            __syncthreads();
+            int before = 0;
+            for (int q = 0; q < w; ++q) before += wc[q];
+            if (pr) it.list[pos + before + __popcll(m & lt)] = (int32_t) id;
+            pos += wc[0] + wc[1] + wc[2] + wc[3];
+            __syncthreads();
+        }
+    }
+}
+
+// flush_acti_buffer of the items that have a pending buffer, then the level
+// histogram of every item in the pool table
+static __global__ void __launch_bounds__(256) k_pool_flush(const Item *__restrict__ items, const int *__restrict__ toff,
+                                                    int n_items) {
+    const int total = toff[n_items];
+    for (int t = blockIdx.x; t < total; t += gridDim.x) {
+        const int tile = xcd_block(t, total);
+        const int i = find_item(toff, n_items, tile);
+        const Item &it = items[i];
+        if (!it.flush) continue;
+        const int64_t base = (int64_t) (tile - toff[i]) * POOL_TILE;
+        for (int k = 0; k < POOL_TILE; k += 256) flush_node(it.d, base + k + threadIdx.x, it.n, it.L);
+    }
+}
+
+static __global__ void k_zero_hist(const Item *__restrict__ items, int n_items) {
+    const int i = blockIdx.x;
+    if (i < n_items && threadIdx.x < 16) items[i].hist[threadIdx.x] = 0;
+}
+
+static __global__ void __launch_bounds__(256) k_pool_hist(const Item *__restrict__ items, const int *__restrict__ toff,
+                                                   int n_items) {
+    __shared__ unsigned int h[16];
+    const int total = toff[n_items];
+    for (int t = blockIdx.x; t < total; t += gridDim.x) {
+        const int tile = xcd_block(t, total);
+        const int i = find_item(toff, n_items, tile);
+        const Item &it = items[i];
+        if (threadIdx.x < 16) h[threadIdx.x] = 0;
+        __syncthreads();
+        const int64_t base = (int64_t) (tile - toff[i]) * POOL_TILE;
+        for (int k = 0; k < POOL_TILE; k += 256) {
+            const int64_t id = base + k + threadIdx.x;
+            if (id >= 1 && id < it.n) {
+                const int cl = it.d.cluster[id];
+                if (cl >= 0 && cl < 16) atomicAdd(&h[cl], 1u);
+            }
+        }
+        __syncthreads();
+        if (threadIdx.x < 16 && h[threadIdx.x]) atomicAdd(&it.hist[threadIdx.x], (unsigned long long) h[threadIdx.x]);
+        __syncthreads();
+    }
 }
 
 }  // namespace dacti

@@ -124,29 +124,6 @@ int build_list(dacti_handle *h, int L, int64_t *nl, int32_t *out) {
     return DACTI_OK;
 }
 
-// cluster-L list without waiting for its size (the host knows it from the
-// level histogram); the device count goes to *count
-int build_list_async(dacti_handle *h, int L, int32_t *out, int32_t *count) {
-    hipcub::CountingInputIterator<int32_t> it(1);
-    size_t bytes = h->tmp_bytes;
-    HIPCHK(h, hipcub::DeviceSelect::If(h->d_tmp, bytes, it, out, count, (int) (h->n - 1), ClusterIs{h->d.cluster, L},
-                                       h->stream));
-    return DACTI_OK;
-}
-
-// level histogram and status in one round trip
-int finish_substep(dacti_handle *h) {
-    HIPCHK(h, hipMemsetAsync(h->d_hist, 0, sizeof(unsigned long long) * 16, h->stream));
-    const int64_t nb = std::min<int64_t>(blocks(h->n, 256), 4096);
-    k_level_hist<<<(unsigned) std::max<int64_t>(nb, 1), 256, 0, h->stream>>>(h->d.cluster, h->n, h->d_hist);
-    HIPCHK(h, hipGetLastError());
-    HIPCHK(h, hipMemcpyAsync(h->h_hist, h->d_hist, sizeof(unsigned long long) * 16, hipMemcpyDeviceToHost, h->stream));
-    HIPCHK(h, hipMemcpyAsync(h->h_status, h->d_status, sizeof(Status), hipMemcpyDeviceToHost, h->stream));
-    HIPCHK(h, hipStreamSynchronize(h->stream));
-    for (int l = 0; l < 16; ++l) h->counts[l] = (int64_t) h->h_hist[l];
-    return DACTI_OK;
-}
-
 int read_status(dacti_handle *h) {
     HIPCHK(h, hipMemcpyAsync(h->h_status, h->d_status, sizeof(Status), hipMemcpyDeviceToHost, h->stream));
     HIPCHK(h, hipStreamSynchronize(h->stream));
@@ -263,17 +240,118 @@ int init_any(dacti_handle *h) {
     return DACTI_E_ARG;
 }
 
-int integrate_any(dacti_handle *h, int signal, int *fb) {
+int batch_integrate_any(dacti_batch *b, int signal, int *fb) {
 #define DACTI_CASE_INT(M, Dm) \
-    if (h->model == M && h->D == Dm) return variant_integrate_##M##_##Dm(h, signal, fb);
+    if (b->model == M && b->D == Dm) return variant_integrate_##M##_##Dm(b, signal, fb);
     DACTI_VARIANTS(DACTI_CASE_INT)
-    h->err = "unsupported model/dimension";
+    b->err = "unsupported model/dimension";
     return DACTI_E_ARG;
+}
+
+int integrate_any(dacti_handle *h, int signal, int *fb) {
+    const int rc = batch_integrate_any(h->solo, signal, fb);
+    if (rc) h->err = h->solo->err;
+    return rc;
+}
+
+// ------------------------------------------------------------------ batches
+void batch_drop_graphs(dacti_batch *b) {
+    if (b->g_head) (void) hipGraphExecDestroy(b->g_head);
+    if (b->g_tail) (void) hipGraphExecDestroy(b->g_tail);
+    b->g_head = b->g_tail = nullptr;
+}
+
+void batch_free(dacti_batch *b) {
+    if (!b) return;
+    if (b->stream) (void) hipStreamSynchronize(b->stream);
+    batch_drop_graphs(b);
+    for (void *p : {(void *) b->d_items, (void *) b->d_mcs, (void *) b->d_toff, (void *) b->d_toffp, (void *) b->d_tilecnt, (void *) b->d_st,
+                    (void *) b->d_hist})
+        if (p) (void) hipFree(p);
+    for (void *p : {(void *) b->h_items, (void *) b->h_toff, (void *) b->h_toffp, (void *) b->h_st, (void *) b->h_hist})
+        if (p) (void) hipHostFree(p);
+    for (auto &e : b->ev)
+        if (e) (void) hipEventDestroy(e);
+    if (b->own_stream && b->stream) (void) hipStreamDestroy(b->stream);
+    delete b;
+}
+
+int batch_ensure_tiles(dacti_batch *b, int64_t tiles) {
+    if (tiles > INT32_MAX / 2) {
+        b->err = "batch too large";
+        return DACTI_E_ARG;
+    }
+    if (tiles <= b->tilecnt_cap) return DACTI_OK;
+    // the captured graphs hold the old table
+    if (hipStreamSynchronize(b->stream) != hipSuccess) return DACTI_E_HIP;
+    batch_drop_graphs(b);
+    if (b->d_tilecnt) (void) hipFree(b->d_tilecnt);
+    b->d_tilecnt = nullptr;
+    const int64_t cap = tiles + tiles / 2 + 64;
+    if (hipMalloc(&b->d_tilecnt, sizeof(int) * (size_t) cap) != hipSuccess) {
+        b->tilecnt_cap = 0;
+        b->err = "tile table allocation failed";
+        return DACTI_E_MEMORY;
+    }
+    b->tilecnt_cap = cap;
+    return DACTI_OK;
+}
+
+// a batch over handles of one model; stream: the stream to run on (a new
+// one when null)
+int batch_make(dacti_handle *const *hs, int n, hipStream_t stream, dacti_batch **out, std::string &err) {
+    dacti_batch *b = new dacti_batch();
+    b->hs.assign(hs, hs + n);
+    b->broken.assign((size_t) n, 0);
+    b->n = n;
+    b->model = hs[0]->model;
+    b->D = hs[0]->D;
+    if (stream) {
+        b->stream = stream;
+    } else {
+        if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) {
+            err = "stream creation failed";
+            delete b;
+            return DACTI_E_HIP;
+        }
+        b->own_stream = true;
+    }
+    const size_t n1 = (size_t) n + 1;
+    bool ok = hipMalloc(&b->d_items, sizeof(Item) * (size_t) n) == hipSuccess &&
+              hipMalloc(&b->d_mcs, sizeof(MConst) * (size_t) n) == hipSuccess &&
+              hipMalloc(&b->d_toff, sizeof(int) * n1) == hipSuccess &&
+              hipMalloc(&b->d_toffp, sizeof(int) * n1) == hipSuccess &&
+              hipMalloc(&b->d_st, sizeof(Status) * (size_t) n) == hipSuccess &&
+              hipMalloc(&b->d_hist, sizeof(unsigned long long) * 16 * (size_t) n) == hipSuccess &&
+              hipHostMalloc(&b->h_items, sizeof(Item) * (size_t) n) == hipSuccess &&
+              hipHostMalloc(&b->h_toff, sizeof(int) * n1) == hipSuccess &&
+              hipHostMalloc(&b->h_toffp, sizeof(int) * n1) == hipSuccess &&
+              hipHostMalloc(&b->h_st, sizeof(Status) * (size_t) n) == hipSuccess &&
+              hipHostMalloc(&b->h_hist, sizeof(unsigned long long) * 16 * (size_t) n) == hipSuccess;
+    for (auto &e : b->ev) ok = ok && hipEventCreate(&e) == hipSuccess;
+    if (!ok) {
+        err = "batch allocation failed";
+        batch_free(b);
+        return DACTI_E_MEMORY;
+    }
+    for (int i = 0; i < n && ok; ++i)
+        ok = hipMemcpy(b->d_mcs + i, &hs[i]->mc, sizeof(MConst), hipMemcpyHostToDevice) == hipSuccess;
+    if (!ok) {
+        err = "constant upload failed";
+        batch_free(b);
+        return DACTI_E_HIP;
+    }
+    std::memset(b->h_toff, 0, sizeof(int) * n1);
+    std::memset(b->h_toffp, 0, sizeof(int) * n1);
+    std::memset((void *) b->h_st, 0, sizeof(Status) * (size_t) n);
+    *out = b;
+    return DACTI_OK;
 }
 
 void destroy_handle(dacti_handle *h) {
     if (!h) return;
     if (h->stream) (void) hipStreamSynchronize(h->stream);
+    batch_free(h->solo);
     free_dev(h, h->d);
     for (void *p : {(void *) h->d_list, (void *) h->d_list2, (void *) h->d_off, (void *) h->d_rn, (void *) h->d_nsel, h->d_tmp,
                     (void *) h->d_hist, (void *) h->d_status})
@@ -424,6 +502,10 @@ int dacti_create(const dacti_case *c, const dacti_tree *t, dacti_handle **out) {
             h->err = "event creation failed";
             return fail(DACTI_E_HIP);
         }
+    {
+        dacti_handle *one[1] = {h};
+        if ((rc = batch_make(one, 1, h->stream, &h->solo, h->err))) return fail(rc);
+    }
     if (hipMalloc(&h->d_nsel, sizeof(int32_t)) != hipSuccess ||
         hipMalloc(&h->d_hist, sizeof(unsigned long long) * 16) != hipSuccess ||
         hipMalloc(&h->d_status, sizeof(Status)) != hipSuccess ||
@@ -550,6 +632,143 @@ int dacti_export_nodes(dacti_handle *h, const dacti_node_view *v) {
     return DACTI_OK;
 }
 
+int dacti_batch_create(dacti_handle *const *handles, int32_t n, dacti_batch **out) {
+    if (!handles || n < 1 || !out) {
+        g_create_error = "dacti_batch_create: invalid arguments";
+        return DACTI_E_ARG;
+    }
+    *out = nullptr;
+    for (int i = 0; i < n; ++i) {
+        if (!handles[i] || handles[i]->model != handles[0]->model || handles[i]->D != handles[0]->D) {
+            g_create_error = "dacti_batch_create: handles must be valid and share model and dimension";
+            return DACTI_E_ARG;
+        }
+        for (int j = 0; j < i; ++j)
+            if (handles[j] == handles[i]) {
+                g_create_error = "dacti_batch_create: a handle appears twice";
+                return DACTI_E_ARG;
+            }
+        if (hipStreamSynchronize(handles[i]->stream) != hipSuccess) {
+            g_create_error = "dacti_batch_create: a handle's stream failed";
+            return DACTI_E_HIP;
+        }
+    }
+    std::string err;
+    const int rc = batch_make(handles, n, nullptr, out, err);
+    if (rc) g_create_error = "dacti_batch_create: " + err;
+    return rc;
+}
+
+int dacti_batch_integrate(dacti_batch *b, int32_t signal, int32_t *force_break) {
+    if (!b) return DACTI_E_ARG;
+    int fb = 0;
+    const int rc = batch_integrate_any(b, signal, &fb);
+    if (force_break) *force_break = fb;
+    return rc;
+}
+
+int dacti_batch_run(dacti_batch *b, int64_t steps, int32_t signal, int64_t *completed) {
+    if (!b || steps < 0) return DACTI_E_ARG;
+    int64_t s = 0;
+    for (; s < steps; ++s) {
+        int fb = 0;
+        const int rc = batch_integrate_any(b, signal, &fb);
+        if (rc) {
+            if (completed) *completed = s;
+            return rc;
+        }
+        if (fb) break;
+    }
+    if (completed) *completed = s;
+    return DACTI_OK;
+}
+
+int dacti_batch_set_launch(dacti_batch *b, int32_t block, int32_t grid, int32_t graphs) {
+    if (!b || (block != 64 && block != 128 && block != 256) || grid < 0 || (grid % 8) != 0 || (graphs && grid == 0)) {
+        if (b) b->err = "dacti_batch_set_launch: block must be 64, 128 or 256, grid a multiple of 8 (non-zero with graphs)";
+        return DACTI_E_ARG;
+    }
+    if (block != b->block || grid != b->grid || (graphs != 0) != b->graphs) batch_drop_graphs(b);
+    b->block = block;
+    b->grid = grid;
+    b->graphs = graphs != 0;
+    b->tuning = false;
+    return DACTI_OK;
+}
+
+int dacti_batch_get_launch(dacti_batch *b, int32_t *block, int32_t *grid, int32_t *graphs) {
+    if (!b) return DACTI_E_ARG;
+    if (block) *block = b->block;
+    if (grid) *grid = b->grid;
+    if (graphs) *graphs = b->graphs ? 1 : 0;
+    return DACTI_OK;
+}
+
+int dacti_batch_autotune(dacti_batch *b) {
+    if (!b) return DACTI_E_ARG;
+    int cus = 256;
+    hipDeviceProp_t prop;
+    int dev = 0;
+    if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess && prop.multiProcessorCount > 0)
+        cus = prop.multiProcessorCount;
+    const int g8 = (cus + 7) / 8 * 8;
+    b->tune_block = {256, 256, 256, 128, 128, 64};
+    b->tune_grid = {0, 4 * g8, 16 * g8, 8 * g8, 32 * g8, 16 * g8};
+    b->tune_graphs = {0, 1, 1, 1, 1, 1};
+    b->tune_ms.assign(b->tune_block.size(), 0.0);
+    b->tune_next = 0;
+    b->tuning = true;
+    return DACTI_OK;
+}
+
+int dacti_batch_tune_results(dacti_batch *b, int32_t *n, int32_t *block, int32_t *grid, int32_t *graphs,
+                             double *ms) {
+    if (!b || !n) return DACTI_E_ARG;
+    const int32_t cap = *n;
+    *n = (int32_t) b->tune_block.size();
+    for (int32_t k = 0; k < *n && k < cap; ++k) {
+        if (block) block[k] = b->tune_block[(size_t) k];
+        if (grid) grid[k] = b->tune_grid[(size_t) k];
+        if (graphs) graphs[k] = b->tune_graphs[(size_t) k];
+        if (ms) ms[k] = b->tune_ms[(size_t) k];
+    }
+    return DACTI_OK;
+}
+
+int dacti_batch_get_stats(dacti_batch *b, dacti_stats *s) {
+    if (!b || !s) return DACTI_E_ARG;
+    std::memset(s, 0, sizeof *s);
+    for (dacti_handle *h : b->hs) {
+        s->n_nodes += h->n;
+        for (int l = 0; l < 16; ++l) s->n_active += h->counts[l] + h->pending[l];
+        s->max_acti_level = std::max(s->max_acti_level, h->max_acti_level);
+    }
+    s->n_substeps = b->n_substeps;
+    s->n_global_steps = b->n_global_steps;
+    s->cell_updates = b->cell_updates;
+    s->flux_launches = b->launches;
+    s->flux_kernel_ms = b->flux_ms;
+    s->substep_kernel_ms = b->substep_ms;
+    s->global_dt = b->hs[0]->global_dt;
+    return DACTI_OK;
+}
+
+int dacti_batch_set_timing(dacti_batch *b, int32_t on) {
+    if (!b) return DACTI_E_ARG;
+    b->timing = on != 0;
+    return DACTI_OK;
+}
+
+int dacti_batch_destroy(dacti_batch *b) {
+    batch_free(b);
+    return DACTI_OK;
+}
+
+const char *dacti_batch_last_error(dacti_batch *b) {
+    if (!b) return g_create_error.c_str();
+    return b->err.c_str();
+}
+
 int dacti_get_stats(dacti_handle *h, dacti_stats *s) {
     if (!h || !s) return DACTI_E_ARG;
     std::memset(s, 0, sizeof *s);
@@ -561,8 +780,8 @@ int dacti_get_stats(dacti_handle *h, dacti_stats *s) {
     s->n_global_steps = h->n_global_steps;
     s->cell_updates = h->cell_updates;
     s->flux_launches = h->flux_launches;
-    s->flux_kernel_ms = h->flux_ms;
-    s->substep_kernel_ms = h->substep_ms;
+    s->flux_kernel_ms = h->solo->flux_ms;
+    s->substep_kernel_ms = h->solo->substep_ms;
     s->global_dt = h->global_dt;
     s->max_acti_level = h->max_acti_level;
     return DACTI_OK;
@@ -571,6 +790,7 @@ int dacti_get_stats(dacti_handle *h, dacti_stats *s) {
 int dacti_set_timing(dacti_handle *h, int32_t on) {
     if (!h) return DACTI_E_ARG;
     h->timing = on != 0;
+    h->solo->timing = on != 0;
     return DACTI_OK;
 }
 

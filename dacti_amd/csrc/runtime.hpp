@@ -11,6 +11,7 @@
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -84,6 +85,46 @@ struct dacti_handle {
     int32_t *h_nsel = nullptr;
     unsigned long long *h_hist = nullptr;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    struct dacti_batch *solo = nullptr;    // batch of this handle alone
+    std::string err;
+};
+
+// A batch: instances stepped together, one launch per op and LTS substep
+// for all of them (see kernels.hpp, batched launches).  A handle alone is
+// stepped as its own batch of one.
+struct dacti_batch {
+    std::vector<dacti_handle *> hs;
+    int model = 0, D = 0, n = 0;
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    Item *h_items = nullptr, *d_items = nullptr;
+    MConst *d_mcs = nullptr;                     // constants of each instance
+    int *h_toff = nullptr, *d_toff = nullptr;    // list tiles, n + 1 entries
+    int *h_toffp = nullptr, *d_toffp = nullptr;  // pool tiles, n + 1 entries
+    int *d_tilecnt = nullptr;
+    int64_t tilecnt_cap = 0;
+    Status *h_st = nullptr, *d_st = nullptr;
+    unsigned long long *h_hist = nullptr, *d_hist = nullptr;
+    // launch configuration: threads per list tile, persistent grid (0 = one
+    // block per tile), HIP graphs for the fixed kernel sequences
+    int block = 256, grid = 0;
+    bool graphs = false;
+    hipGraphExec_t g_head = nullptr, g_tail = nullptr;
+    bool timing = false;
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    double flux_ms = 0, substep_ms = 0;
+    int64_t cell_updates = 0, n_substeps = 0, n_global_steps = 0, launches = 0;
+    int top = 15;  // no cluster above this level in any instance
+    // instances whose update produced a NaN state: they stop where the
+    // reference's integrate() returns (impl/integrate.hpp:50) and are not
+    // stepped again (a batch of one restarts them on the next call, as the
+    // reference does)
+    std::vector<char> broken;
+    // autotuner: candidate launch configurations, one timed global step each
+    std::vector<int> tune_block, tune_grid, tune_graphs;
+    std::vector<double> tune_ms;
+    size_t tune_next = 0;
+    bool tuning = false;
     std::string err;
 };
 
@@ -109,17 +150,47 @@ struct ClusterIs {
 int alloc_pool(dacti_handle *h, int64_t C, Dev *old, int64_t n);
 int ensure_capacity(dacti_handle *h, int64_t need);
 int build_list(dacti_handle *h, int L, int64_t *nl, int32_t *out = nullptr);
-int build_list_async(dacti_handle *h, int L, int32_t *out, int32_t *count);
 int read_status(dacti_handle *h);
 int refresh_counts(dacti_handle *h);
-int finish_substep(dacti_handle *h);
 int gc(dacti_handle *h);
+int batch_ensure_tiles(dacti_batch *b, int64_t tiles);
+void batch_drop_graphs(dacti_batch *b);
+
+// Makes a handle use its batch's stream and status slot while the batch
+// steps it (remesh and gc run on the handle).
+struct Bind {
+    dacti_handle *h;
+    hipStream_t s0;
+    Status *d0, *h0;
+    Bind(dacti_batch *b, int i) : h(b->hs[(size_t) i]), s0(h->stream), d0(h->d_status), h0(h->h_status) {
+        h->stream = b->stream;
+        h->d_status = b->d_st + i;
+        h->h_status = b->h_st + i;
+    }
+    ~Bind() {
+        h->stream = s0;
+        h->d_status = d0;
+        h->h_status = h0;
+    }
+};
+
+inline Item base_item(dacti_batch *b, int i) {
+    dacti_handle *h = b->hs[(size_t) i];
+    Item it{};
+    it.d = h->d;
+    it.list = h->d_list;
+    it.rest = h->d_list2;
+    it.st = b->d_st + i;
+    it.hist = b->d_hist + 16 * i;
+    it.n = h->n;
+    return it;
+}
 
 // per-model entry points, one translation unit each (variant_*.hip)
 #define DACTI_VARIANTS(X) X(EULER, 2) X(EULER, 3) X(NAVIER_STOKES, 2) X(NAVIER_STOKES, 3) X(MULTI, 2) X(MULTI, 3)
 #define DACTI_DECLARE_VARIANT(M, Dm) \
     int variant_init_##M##_##Dm(dacti_handle *h); \
-    int variant_integrate_##M##_##Dm(dacti_handle *h, int signal, int *force_break);
+    int variant_integrate_##M##_##Dm(dacti_batch *b, int signal, int *force_break);
 DACTI_VARIANTS(DACTI_DECLARE_VARIANT)
 
 template <int MODEL, int D>
@@ -175,195 +246,35 @@ struct Ops {
         return DACTI_OK;
     }
 
-    // multiphase interface fields: impl/integrate.hpp:49-54
+    // multiphase interface fields over h->d_list[0, nl) (constructor,
+    // integrator.hpp:130-140), through the batched ops of the handle's batch
     static int chain(dacti_handle *h, int nl) {
         if constexpr (MODEL == MULTI) {
-            Dev &d = h->d;
+            dacti_batch *b = h->solo;
+            b->h_items[0] = base_item(b, 0);
+            b->h_items[0].nl = nl;
+            b->h_toff[0] = 0;
+            b->h_toff[1] = (int) blocks(nl, b->block);
+            HIPCHK(h, hipMemcpyAsync(b->d_items, b->h_items, sizeof(Item), hipMemcpyHostToDevice, h->stream));
+            HIPCHK(h, hipMemcpyAsync(b->d_toff, b->h_toff, 2 * sizeof(int), hipMemcpyHostToDevice, h->stream));
+            HIPCHK(h, hipMemsetAsync(b->d_st, 0, sizeof(Status), h->stream));
+            const unsigned g = std::max(1u, (unsigned) b->h_toff[1]);
             for (int it = 0; it < 3; ++it) {
-                k_smooth_alpha<MODEL, D><<<blocks(nl), 256, 0, h->stream>>>(d, h->mc, h->d_list, nl);
-                k_store_k<0><<<blocks(nl), 256, 0, h->stream>>>(d, h->d_list, nl, S::IALPHAS);
+                k_batch<<<g, b->block, 0, h->stream>>>(b->d_items, b->d_mcs, b->d_toff, 1, OpSmoothAlpha<MODEL, D>{});
+                k_batch<<<g, b->block, 0, h->stream>>>(b->d_items, b->d_mcs, b->d_toff, 1, OpStoreK{S::IALPHAS});
             }
-            k_normal<MODEL, D><<<blocks(nl), 256, 0, h->stream>>>(d, h->mc, h->d_list, nl);
-            k_curvature<MODEL, D><<<blocks(nl), 256, 0, h->stream>>>(d, h->mc, h->d_list, nl);
-            k_smooth_curvature<MODEL, D><<<blocks(nl), 256, 0, h->stream>>>(d, h->mc, h->d_list, nl);
-            k_store_k<0><<<blocks(nl), 256, 0, h->stream>>>(d, h->d_list, nl, S::IKAPPA);
+            k_batch<<<g, b->block, 0, h->stream>>>(b->d_items, b->d_mcs, b->d_toff, 1, OpNormal<MODEL, D>{});
+            k_batch<<<g, b->block, 0, h->stream>>>(b->d_items, b->d_mcs, b->d_toff, 1, OpCurvature<MODEL, D>{});
+            k_batch<<<g, b->block, 0, h->stream>>>(b->d_items, b->d_mcs, b->d_toff, 1, OpSmoothCurvature<MODEL, D>{});
+            k_batch<<<g, b->block, 0, h->stream>>>(b->d_items, b->d_mcs, b->d_toff, 1, OpStoreK{S::IKAPPA});
             HIPCHK(h, hipGetLastError());
-        }
-        return DACTI_OK;
-    }
-
-    // one level substep: impl/integrate.hpp:42-64; returns 1 on force_break
-    static int substep(dacti_handle *h, int L, int signal, int *force_break) {
-        const double dt = h->global_dt / (1 << L);
-        // the cluster size is known from the level histogram of the last substep
-        const int nl = (int) h->counts[L];
-        int rc = 0;
-        bool changed = false;
-        if (nl > 0) {
-            HIPCHK(h, hipMemsetAsync(h->d_status, 0, sizeof(Status), h->stream));
-            rc = build_list_async(h, L, h->d_list, &h->d_status->n_list);
-            if (rc) return rc;
-            if (h->timing) HIPCHK(h, hipEventRecord(h->ev[0], h->stream));
-            k_flux_simple<MODEL, D><<<blocks(nl), 256, 0, h->stream>>>(h->d, h->mc, h->d_list, nl, L, dt, h->d_list2,
-                                                                       &h->d_status->n_rest);
-            k_flux<MODEL, D><<<std::min<unsigned>(blocks(nl), 1024), 256, 0, h->stream>>>(h->d, h->mc, h->d_list2,
-                                                                                      &h->d_status->n_rest, L, dt);
-            k_update<MODEL, D><<<blocks(nl), 256, 0, h->stream>>>(h->d, h->mc, h->d_list, nl, dt, signal, h->d_status);
-            HIPCHK(h, hipGetLastError());
-            if (h->timing) HIPCHK(h, hipEventRecord(h->ev[1], h->stream));
-            h->flux_launches++;
-            h->cell_updates += nl;
-            if constexpr (MODEL == MULTI) {
-                // the interface chain is not guarded by the NaN flag: stop here
-                rc = read_status(h);
-                if (rc) return rc;
-                if (h->h_status->nan_flag) {
-                    *force_break = 1;
-                    return 1;
-                }
-            }
-            // slopes, refinement marks and LTS decisions in one sweep (a
-            // no-op when the update produced a NaN state)
-            k_sweep<MODEL, D><<<blocks(nl), 256, 0, h->stream>>>(h->d, h->mc, h->d_list, nl, L, dt,
-                                                                 h->acti_counter[L], 0, h->d_status);
-            HIPCHK(h, hipGetLastError());
-            rc = chain(h, nl);
-            if (rc) return rc;
-            rc = read_status(h);
-            if (rc) return rc;
-            if (h->h_status->nan_flag) {
-                *force_break = 1;
-                return 1;
-            }
-            if (h->h_status->n_list != nl) {
-                h->err = "internal: cluster list size differs from the level histogram";
-                return DACTI_E_STATE;
-            }
-            if (h->h_status->n_deferred > 0) {
-                k_sweep<MODEL, D><<<blocks(nl), 256, 0, h->stream>>>(h->d, h->mc, h->d_list, nl, L, dt,
-                                                                     h->acti_counter[L], 1, h->d_status);
-                HIPCHK(h, hipGetLastError());
-                rc = read_status(h);
-                if (rc) return rc;
-            }
-            if (h->h_status->n_marked > 0) {
-                rc = remesh(h, L, nl, dt);
-                if (rc) return rc;
-            }
-            k_apply<<<blocks(nl), 256, 0, h->stream>>>(h->d, h->d_list, nl, L);
-            HIPCHK(h, hipGetLastError());
-            if (h->timing) HIPCHK(h, hipEventRecord(h->ev[2], h->stream));
-            changed = true;
-        }
-        h->acti_counter[L] = (h->acti_counter[L] + 1) % 2;
-        h->acti_time[L] += h->global_dt / (1 << L);
-        if (h->pending[L] > 0) {
-            k_flush<<<blocks(h->n), 256, 0, h->stream>>>(h->d, h->n, L);
-            HIPCHK(h, hipGetLastError());
-            h->pending[L] = 0;
-            changed = true;
-        }
-        if (changed) {
-            // one round trip: level histogram, status (dev_error, lowered)
-            rc = finish_substep(h);
-            if (rc) return rc;
-        }
-        if (nl > 0) {
-            if (h->h_status->dev_error) {
-                h->err = "a neighbour subtree is more than three levels deeper than the cell (2:1 balance broken)";
-                return DACTI_E_STATE;
-            }
-            if (h->timing) {
-                float a = 0, b = 0;
-                HIPCHK(h, hipEventElapsedTime(&a, h->ev[0], h->ev[1]));
-                HIPCHK(h, hipEventElapsedTime(&b, h->ev[0], h->ev[2]));
-                h->flux_ms += a;
-                h->substep_ms += b;
-            }
-            if (L > 0) h->pending[L - 1] += h->h_status->lowered;
-        }
-        h->n_substeps++;
-        return 0;
-    }
-
-    // split / unsplit application and neighbour update (:97-262), then the
-    // LTS decision of the cells that were marked or created
-    static int remesh(dacti_handle *h, int L, int nl, double dt) {
-        k_mark2<D><<<blocks(nl), 256, 0, h->stream>>>(h->d, h->d_list, nl, L, h->d_rn, h->d_status);
-        HIPCHK(h, hipGetLastError());
-        int rc = read_status(h);
-        if (rc) return rc;
-        const int n_ready = h->h_status->n_ready_split;
-        const int n_new = h->h_status->n_new_blocks;
-        bool changed = false;
-        if (n_ready > 0) {
-            size_t bytes = h->tmp_bytes;
-            HIPCHK(h, hipcub::DeviceScan::ExclusiveSum(h->d_tmp, bytes, h->d_rn, h->d_off, nl, h->stream));
-            const int64_t need = h->n + (int64_t) S::NCH * n_new;
-            if (need > h->d.C) {
-                // keep the list across the reallocation
-                std::vector<int32_t> keep((size_t) nl), offs((size_t) nl);
-                HIPCHK(h, hipMemcpyAsync(keep.data(), h->d_list, sizeof(int32_t) * nl, hipMemcpyDeviceToHost, h->stream));
-                HIPCHK(h, hipMemcpyAsync(offs.data(), h->d_off, sizeof(int32_t) * nl, hipMemcpyDeviceToHost, h->stream));
-                HIPCHK(h, hipStreamSynchronize(h->stream));
-                rc = ensure_capacity(h, need + need / 4);
-                if (rc) return rc;
-                HIPCHK(h, hipMemcpyAsync(h->d_list, keep.data(), sizeof(int32_t) * nl, hipMemcpyHostToDevice, h->stream));
-                HIPCHK(h, hipMemcpyAsync(h->d_off, offs.data(), sizeof(int32_t) * nl, hipMemcpyHostToDevice, h->stream));
-                HIPCHK(h, hipStreamSynchronize(h->stream));
-            }
-            k_split_slopes<MODEL, D><<<blocks(nl), 256, 0, h->stream>>>(h->d, h->mc, h->d_list, nl);
-            k_split<MODEL, D><<<blocks(nl), 256, 0, h->stream>>>(h->d, h->mc, h->d_list, nl, h->d_off, h->n, L);
-            HIPCHK(h, hipGetLastError());
-            h->n += (int64_t) S::NCH * n_new;
-            changed = true;
-        }
-        k_unsplit<MODEL, D><<<blocks(nl), 256, 0, h->stream>>>(h->d, h->mc, h->d_list, nl, L, h->d_status);
-        HIPCHK(h, hipGetLastError());
-        rc = read_status(h);
-        if (rc) return rc;
-        if (h->h_status->n_unsplit > 0) changed = true;
-        if (changed) {
-            k_neighbors<D><<<blocks(h->n), 256, 0, h->stream>>>(h->d, h->mc, h->n);
-            HIPCHK(h, hipGetLastError());
-        }
-        // cluster L now: the listed cells (split parents and coarsened
-        // children still in it, to be dropped), new children, unsplit parents
-        int64_t n2 = 0;
-        rc = build_list(h, L, &n2, h->d_list2);
-        if (rc) return rc;
-        if (n2 > 0) {
-            k_reassign<MODEL, D><<<blocks(n2), 256, 0, h->stream>>>(h->d, h->mc, h->d_list2, (int) n2, L, dt,
-                                                                     h->acti_counter[L], h->d_status);
-            HIPCHK(h, hipGetLastError());
+            HIPCHK(h, hipStreamSynchronize(h->stream));
         }
         return DACTI_OK;
     }
 };
 
-template <int MODEL, int D>
-int integrate_impl(dacti_handle *h, int signal, int *force_break) {
-    *force_break = 0;
-    for (int seq = 1; seq <= (1 << 15); ++seq) {
-        for (int l = 0; l < 16; ++l) {
-            if (seq % (1 << l) != 0) break;
-            const int L = 15 - l;
-            if (L > h->max_acti_level && h->counts[L] > 0) {
-                h->max_acti_level = L;
-                h->acti_time[L] = h->acti_time[L - 1];
-            }
-            if (L > h->max_acti_level) continue;
-            const int rc = Ops<MODEL, D>::substep(h, L, signal, force_break);
-            if (rc < 0) return rc;
-            if (rc == 1) return DACTI_OK;
-        }
-    }
-    int64_t nact = 0;
-    for (int l = 0; l < 16; ++l) nact += h->counts[l];
-    h->n_global_steps++;
-    if ((double) nact / (double) (h->n - 1) < 0.2) return gc(h);
-    return DACTI_OK;
-}
+#include "batch.hpp"
 
 template <int MODEL, int D>
 int dispatch_init(dacti_handle *h) {

@@ -1,10 +1,10 @@
-// variant_navier_stokes2.hip -- the leaf-update kernels and substep driver for one
+// variant_navier_stokes2.hip -- the leaf-update kernels and the batched substep driver for one
 // model (navier_stokes, 2-D), compiled as a translation unit of its own.
 #include "runtime.hpp"
 
 namespace dacti_rt {
 int variant_init_NAVIER_STOKES_2(dacti_handle *h) { return Ops<NAVIER_STOKES, 2>::init(h); }
-int variant_integrate_NAVIER_STOKES_2(dacti_handle *h, int signal, int *force_break) {
-    return integrate_impl<NAVIER_STOKES, 2>(h, signal, force_break);
+int variant_integrate_NAVIER_STOKES_2(dacti_batch *b, int signal, int *force_break) {
+    return BOps<NAVIER_STOKES, 2>::integrate(b, signal, force_break);
 }
 }  // namespace dacti_rt

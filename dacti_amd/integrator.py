@@ -118,3 +118,84 @@ class Integrator:
 
     def __exit__(self, *exc):
         self.close()
+
+
+class Batch:
+    """Independent integrators of one model stepped together (dacti_batch_*).
+
+    Every op of an LTS substep is one launch over the cluster lists of all
+    instances; each instance takes part exactly when its own integrate()
+    would run that substep, so it evolves as it would alone.  While the batch
+    exists, step the integrators only through it; their read side
+    (get_nodes, stats, n_active_cells) stays valid.
+    """
+
+    def __init__(self, integrators):
+        self.members = list(integrators)
+        self.lib = _lib.load()
+        arr = (C.c_void_p * len(self.members))(*[g.h for g in self.members])
+        b = C.c_void_p()
+        rc = self.lib.dacti_batch_create(arr, len(self.members), C.byref(b))
+        if rc != 0:
+            raise RuntimeError(f"dacti_batch_create failed ({rc}): {self.lib.dacti_batch_last_error(None).decode()}")
+        self.b = b
+
+    def _check(self, rc, what):
+        if rc != 0:
+            raise RuntimeError(f"{what} failed ({rc}): {self.lib.dacti_batch_last_error(self.b).decode()}")
+
+    def integrate(self, signal=True):
+        """One global step of every instance; returns force_break."""
+        fb = C.c_int32()
+        self._check(self.lib.dacti_batch_integrate(self.b, int(bool(signal)), C.byref(fb)), "batch integrate")
+        return bool(fb.value)
+
+    def run(self, steps, signal=True):
+        done = C.c_int64()
+        self._check(self.lib.dacti_batch_run(self.b, steps, int(bool(signal)), C.byref(done)), "batch run")
+        return done.value
+
+    def set_launch(self, block=256, grid=0, graphs=False):
+        self._check(self.lib.dacti_batch_set_launch(self.b, block, grid, int(bool(graphs))), "set_launch")
+
+    def launch(self):
+        b, g, gr = C.c_int32(), C.c_int32(), C.c_int32()
+        self._check(self.lib.dacti_batch_get_launch(self.b, C.byref(b), C.byref(g), C.byref(gr)), "get_launch")
+        return b.value, g.value, bool(gr.value)
+
+    def autotune(self):
+        """Time one candidate launch configuration per following global step
+        and keep the fastest (the results do not depend on it)."""
+        self._check(self.lib.dacti_batch_autotune(self.b), "autotune")
+
+    def tune_results(self):
+        n = C.c_int32(16)
+        bl, gr, gp = (C.c_int32 * 16)(), (C.c_int32 * 16)(), (C.c_int32 * 16)()
+        ms = (C.c_double * 16)()
+        self._check(self.lib.dacti_batch_tune_results(self.b, C.byref(n), bl, gr, gp, ms), "tune_results")
+        return [(bl[k], gr[k], bool(gp[k]), ms[k]) for k in range(min(n.value, 16))]
+
+    def stats(self):
+        s = _lib.StatsStruct()
+        self._check(self.lib.dacti_batch_get_stats(self.b, C.byref(s)), "batch stats")
+        return s
+
+    def set_timing(self, on):
+        self._check(self.lib.dacti_batch_set_timing(self.b, int(bool(on))), "batch timing")
+
+    def close(self):
+        if getattr(self, "b", None):
+            self.lib.dacti_batch_destroy(self.b)
+            self.b = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()

@@ -176,3 +176,25 @@ def shock_bubble(base=6, extra=4, mach=1.22, radius=0.25, bubble_x=-0.3, shock_x
                                     shock_x=repr(float(shock_x)), r2=repr(r2), u2=repr(u2), p2=repr(p2),
                                     radius=repr(float(radius)), bx=repr(float(bubble_x)))
     return loads_case(toml, "multi", 2)
+
+
+def shock_bubble_family(n, seed=0, base=6, extra_range=(4, 7), mach_range=(1.1, 3.0), radius_range=(0.1, 0.3),
+                        ratio_range=(0.1, 10.0)):
+    """n independent 2D shock-bubble cases for batched runs.
+
+    Instance k draws, from a generator seeded with (seed, k): the shock Mach
+    number uniform in mach_range, the bubble radius uniform in radius_range,
+    the bubble/ambient density ratio log-uniform in ratio_range and the number
+    of refinement levels above the base uniform in extra_range (inclusive).
+    Returns a list of (case, params) pairs.
+    """
+    out = []
+    for k in range(n):
+        g = np.random.default_rng([seed, k])
+        mach = float(g.uniform(*mach_range))
+        radius = float(g.uniform(*radius_range))
+        ratio = float(np.exp(g.uniform(np.log(ratio_range[0]), np.log(ratio_range[1]))))
+        extra = int(g.integers(extra_range[0], extra_range[1] + 1))
+        case = shock_bubble(base=base, extra=extra, mach=mach, radius=radius, rho2=ratio)
+        out.append((case, {"mach": mach, "radius": radius, "ratio": ratio, "extra": extra}))
+    return out

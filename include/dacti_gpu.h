@@ -182,6 +182,48 @@ int dacti_destroy(dacti_handle *h);
  * h is NULL); never NULL */
 const char *dacti_last_error(dacti_handle *h);
 
+/*
+ * Batches.  The reference runs one integrator per process, its model
+ * constants being process-global statics (model/multi.hpp:50-62); it has no
+ * batch API.  A batch steps many independent handles of one model and
+ * dimension together: every op of an LTS substep is one launch over the
+ * cluster lists of all instances, each instance taking part exactly when
+ * its own Integrator::integrate() (impl/integrate.hpp:11-87) would run that
+ * substep, so each evolves as it would alone (same results, bit for bit).
+ * While a batch exists its handles must be stepped only through it; their
+ * read-side calls (export, stats, n_active) stay valid.  Destroying the
+ * batch leaves the handles intact.
+ */
+typedef struct dacti_batch dacti_batch;
+
+int dacti_batch_create(dacti_handle *const *handles, int32_t n, dacti_batch **out);
+
+/* one global step of every instance; *force_break = 1 when an instance
+ * produced a NaN state (the batch stops in that substep) */
+int dacti_batch_integrate(dacti_batch *b, int32_t signal, int32_t *force_break);
+int dacti_batch_run(dacti_batch *b, int64_t steps, int32_t signal, int64_t *completed);
+
+/* launch configuration: block = list entries per tile (64, 128 or 256),
+ * grid = persistent grid in blocks (a multiple of 8; 0 = one block per
+ * tile), graphs = replay the fixed kernel sequences of a substep from HIP
+ * graphs (needs grid > 0).  The results do not depend on it. */
+int dacti_batch_set_launch(dacti_batch *b, int32_t block, int32_t grid, int32_t graphs);
+int dacti_batch_get_launch(dacti_batch *b, int32_t *block, int32_t *grid, int32_t *graphs);
+
+/* autotuner: the next global steps each run one candidate configuration and
+ * are timed; after the last candidate the fastest is kept.  Tuning runs on
+ * the real steps, which advance the state as usual. */
+int dacti_batch_autotune(dacti_batch *b);
+/* candidates and their step times; *n: capacity in, count out */
+int dacti_batch_tune_results(dacti_batch *b, int32_t *n, int32_t *block, int32_t *grid, int32_t *graphs,
+                             double *ms);
+
+/* totals over the batch (cell_updates, substeps, kernel time when timing) */
+int dacti_batch_get_stats(dacti_batch *b, dacti_stats *s);
+int dacti_batch_set_timing(dacti_batch *b, int32_t on);
+int dacti_batch_destroy(dacti_batch *b);
+const char *dacti_batch_last_error(dacti_batch *b);
+
 #ifdef __cplusplus
 }
 #endif

@@ -67,3 +67,11 @@ def test_parity_navier_stokes_2d_amr():
 def test_parity_multi2d_shock_bubble_small():
     case = workloads.shock_bubble(base=4, extra=2)
     _run_pair(case, 4, "multi2d shock-bubble 16x8+2")
+
+
+def test_parity_multi2d_deep_subtrees():
+    # five levels of refinement above the base: the far side of a neighbour
+    # subtree reaches more than three levels below the neighbour, which the
+    # interpolation walk must follow like the reference's recursion
+    case = workloads.shock_bubble(base=4, extra=5, mach=2.0, radius=0.2, rho2=0.5)
+    _run_pair(case, 2, "multi2d shock-bubble 16x8+5")
