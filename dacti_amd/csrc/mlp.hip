@@ -1,0 +1,628 @@
+// mlp.hip -- refinement-predictor MLP: BF16 MFMA GEMMs with FP32
+// accumulation for the forward pass, the backward pass and Adam, all on the
+// GPU.  C ABI in include/dacti_mlp.h.
+//
+// Every layer dimension is padded to a multiple of 16 with zero weights, so
+// every GEMM is a grid of 16x16 output tiles; one wavefront computes one tile
+// with v_mfma_f32_16x16x16_bf16 over 16-deep slices of the reduction.  The
+// operands are addressed through strides, so the forward product
+// (activations x W^T), the input gradient (dZ x W) and the weight gradient
+// (dZ^T x activations) share one kernel.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <random>
+#include <string>
+#include <vector>
+
+#include "../../include/dacti_mlp.h"
+
+namespace dmlp {
+
+typedef short s4 __attribute__((ext_vector_type(4)));
+typedef float f4 __attribute__((ext_vector_type(4)));
+
+// BF16 from FP32: round to nearest, ties to even, on the FP32 bit pattern
+// (a NaN stays a quiet NaN).
+__host__ __device__ inline uint16_t to_bf16(float f) {
+    uint32_t u = __builtin_bit_cast(uint32_t, f);
+    if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t) ((u >> 16) | 0x40u);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (uint16_t) (u >> 16);
+}
+
+inline int pad16(int x) { return (x + 15) / 16 * 16; }
+
+// C(m, n) = sum_k A(m, k) B(k, n), A(m, k) = A[m*sam + k*sak], B(k, n) =
+// B[k*sbk + n*sbn] (BF16 bits); M, N, K multiples of 16.  Epilogue: + bias[n]
+// (if given), ReLU (if relu), FP32 store to C[m*ldc + n], BF16 copy to Cb.
+// Lane layout of the 16x16x16 BF16 MFMA: lane l holds A(l%16, 4(l/16)+j) and
+// B(4(l/16)+j, l%16) for j < 4, and returns C(4(l/16)+i, l%16) for i < 4.
+__global__ void __launch_bounds__(256) k_gemm(const uint16_t *__restrict__ A, int64_t sam, int64_t sak,
+                                              const uint16_t *__restrict__ B, int64_t sbk, int64_t sbn, int M, int N,
+                                              int K, float *__restrict__ C, int64_t ldc, uint16_t *__restrict__ Cb,
+                                              const float *__restrict__ bias, int relu) {
+    const int wave = (int) ((blockIdx.x * (int64_t) blockDim.x + threadIdx.x) >> 6);
+    const int lane = threadIdx.x & 63;
+    const int tn = N / 16, tiles = (M / 16) * tn;
+    if (wave >= tiles) return;
+    const int m0 = (wave / tn) * 16, n0 = (wave % tn) * 16;
+    const int r = lane & 15, kq = (lane >> 4) * 4;
+    f4 c = {0.0f, 0.0f, 0.0f, 0.0f};
+    const uint16_t *pa = A + (int64_t) (m0 + r) * sam + (int64_t) kq * sak;
+    const uint16_t *pb = B + (int64_t) kq * sbk + (int64_t) (n0 + r) * sbn;
+    for (int k0 = 0; k0 < K; k0 += 16) {
+        s4 a, b;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            a[j] = (short) pa[(int64_t) (k0 + j) * sak];
+            b[j] = (short) pb[(int64_t) (k0 + j) * sbk];
+        }
+        c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const int m = m0 + kq + i, n = n0 + r;
+        float v = c[i];
+        if (bias) v += bias[n];
+        if (relu) v = v > 0.0f ? v : 0.0f;
+        C[(int64_t) m * ldc + n] = v;
+        if (Cb) Cb[(int64_t) m * ldc + n] = to_bf16(v);
+    }
+}
+
+__global__ void k_to_bf16(const float *__restrict__ x, uint16_t *__restrict__ y, int64_t n) {
+    const int64_t i = blockIdx.x * (int64_t) blockDim.x + threadIdx.x;
+    if (i < n) y[i] = to_bf16(x[i]);
+}
+
+// rows of the padded BF16 input: row r <- sample idx[r] (r < nb), zero beyond
+__global__ void k_gather(const float *__restrict__ x, int64_t ldx, int nin, const int32_t *__restrict__ idx, int nb,
+                         int rows, uint16_t *__restrict__ xb, int ldb, const int32_t *__restrict__ y,
+                         int32_t *__restrict__ yb) {
+    const int r = blockIdx.x, c = threadIdx.x;
+    if (r >= rows) return;
+    const int64_t s = r < nb ? idx[r] : -1;
+    for (int k = c; k < ldb; k += blockDim.x) xb[(int64_t) r * ldb + k] = (s >= 0 && k < nin) ? to_bf16(x[s * ldx + k]) : 0;
+    if (c == 0) yb[r] = (s >= 0 && y) ? y[s] : -1;
+}
+
+// softmax cross-entropy over the first nout columns; dZ = (softmax - onehot) / nb
+__global__ void k_softmax_ce(const float *__restrict__ z, int ldz, int nout, const int32_t *__restrict__ yb, int rows,
+                             float inv_nb, float *__restrict__ dz, uint16_t *__restrict__ dz16,
+                             float *__restrict__ loss) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= rows) return;
+    const int y = yb[r];
+    const float *zr = z + (int64_t) r * ldz;
+    float *dr = dz + (int64_t) r * ldz;
+    uint16_t *d16 = dz16 + (int64_t) r * ldz;
+    if (y < 0) {
+        for (int c = 0; c < ldz; ++c) {
+            dr[c] = 0.0f;
+            d16[c] = 0;
+        }
+        loss[r] = 0.0f;
+        return;
+    }
+    float mx = zr[0];
+    for (int c = 1; c < nout; ++c) mx = fmaxf(mx, zr[c]);
+    float s = 0.0f;
+    for (int c = 0; c < nout; ++c) s += expf(zr[c] - mx);
+    loss[r] = (mx + logf(s)) - zr[y];
+    for (int c = 0; c < ldz; ++c) {
+        float g = 0.0f;
+        if (c < nout) g = (expf(zr[c] - mx) / s - (c == y ? 1.0f : 0.0f)) * inv_nb;
+        dr[c] = g;
+        d16[c] = to_bf16(g);
+    }
+}
+
+// db[o] = sum over rows of dz[r][o] (fixed order)
+__global__ void k_colsum(const float *__restrict__ dz, int ld, int rows, int ncol, float *__restrict__ out) {
+    const int o = blockIdx.x * blockDim.x + threadIdx.x;
+    if (o >= ncol) return;
+    float s = 0.0f;
+    for (int r = 0; r < rows; ++r) s += dz[(int64_t) r * ld + o];
+    out[o] = s;
+}
+
+// dZ_prev = dA * (post-activation > 0), FP32 and BF16
+__global__ void k_relu_back(const float *__restrict__ da, const float *__restrict__ act, int64_t n,
+                            float *__restrict__ dz, uint16_t *__restrict__ dz16) {
+    const int64_t i = blockIdx.x * (int64_t) blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float g = act[i] > 0.0f ? da[i] : 0.0f;
+    dz[i] = g;
+    dz16[i] = to_bf16(g);
+}
+
+__global__ void k_adam(float *__restrict__ w, const float *__restrict__ g, float *__restrict__ m,
+                       float *__restrict__ v, int64_t n, float lr, float c1, float c2) {
+    const int64_t i = blockIdx.x * (int64_t) blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float gi = g[i];
+    const float mi = 0.9f * m[i] + 0.1f * gi;
+    const float vi = 0.999f * v[i] + 0.001f * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    w[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + 1e-8f);
+}
+
+// per-sample losses of a batch scattered to their sample slots
+__global__ void k_scatter_loss(const float *__restrict__ lb, const int32_t *__restrict__ idx, int nb,
+                               float *__restrict__ all) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r < nb) all[idx[r]] = lb[r];
+}
+
+inline unsigned nblk(int64_t n, int b = 256) { return (unsigned) ((n + b - 1) / b); }
+
+}  // namespace dmlp
+
+using namespace dmlp;
+
+struct dacti_mlp {
+    int n_in = 0, H = 0, NL = 0, n_out = 0, L = 0;
+    std::vector<int> ir, orr, ip, op;  // real and padded in/out widths per layer
+    std::vector<float *> W, b, gW, gb, mW, vW, mb, vb;
+    std::vector<uint16_t *> W16;
+    int cap = 0;  // rows of the batch buffers
+    uint16_t *X16 = nullptr;
+    std::vector<float *> Z;
+    std::vector<uint16_t *> A16;
+    float *dZ = nullptr, *dA = nullptr, *lossb = nullptr;
+    uint16_t *dZ16 = nullptr;
+    int32_t *yb = nullptr;
+    hipStream_t s = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    double train_ms = 0, train_flops = 0, flops = 0;
+    int64_t t = 0;  // Adam steps
+    std::string err;
+};
+
+namespace {
+
+thread_local std::string g_err;
+
+#define MCHK(m, call)                                                        \
+    do {                                                                     \
+        hipError_t e_ = (call);                                              \
+        if (e_ != hipSuccess) {                                              \
+            (m)->err = std::string(#call) + ": " + hipGetErrorString(e_);    \
+            return -2;                                                       \
+        }                                                                    \
+    } while (0)
+
+template <typename T>
+int dalloc(dacti_mlp *m, T **p, size_t n) {
+    MCHK(m, hipMalloc((void **) p, sizeof(T) * std::max<size_t>(n, 1)));
+    MCHK(m, hipMemset(*p, 0, sizeof(T) * std::max<size_t>(n, 1)));
+    return 0;
+}
+
+int ensure_rows(dacti_mlp *m, int rows) {
+    if (rows <= m->cap) return 0;
+    MCHK(m, hipStreamSynchronize(m->s));
+    auto fr = [](void *p) {
+        if (p) (void) hipFree(p);
+    };
+    fr(m->X16), fr(m->dZ), fr(m->dA), fr(m->dZ16), fr(m->yb), fr(m->lossb);
+    for (auto p : m->Z) fr(p);
+    for (auto p : m->A16) fr(p);
+    const int cap = pad16(rows);
+    int maxw = 16;
+    for (int l = 0; l < m->L; ++l) maxw = std::max({maxw, m->ip[l], m->op[l]});
+    int rc;
+    if ((rc = dalloc(m, &m->X16, (size_t) cap * m->ip[0])) || (rc = dalloc(m, &m->dZ, (size_t) cap * maxw)) ||
+        (rc = dalloc(m, &m->dA, (size_t) cap * maxw)) || (rc = dalloc(m, &m->dZ16, (size_t) cap * maxw)) ||
+        (rc = dalloc(m, &m->yb, (size_t) cap)) ||
+        (rc = dalloc(m, &m->lossb, (size_t) cap)))
+        return rc;
+    m->Z.assign(m->L, nullptr);
+    m->A16.assign(m->L, nullptr);
+    for (int l = 0; l < m->L; ++l)
+        if ((rc = dalloc(m, &m->Z[l], (size_t) cap * m->op[l])) || (rc = dalloc(m, &m->A16[l], (size_t) cap * m->op[l])))
+            return rc;
+    m->cap = cap;
+    return 0;
+}
+
+int gemm(dacti_mlp *m, const uint16_t *A, int64_t sam, int64_t sak, const uint16_t *B, int64_t sbk, int64_t sbn, int M,
+         int N, int K, float *C, int64_t ldc, uint16_t *Cb, const float *bias, int relu) {
+    const int64_t waves = (int64_t) (M / 16) * (N / 16);
+    k_gemm<<<nblk(waves * 64), 256, 0, m->s>>>(A, sam, sak, B, sbk, sbn, M, N, K, C, ldc, Cb, bias, relu);
+    m->flops += 2.0 * M * N * K;
+    MCHK(m, hipGetLastError());
+    return 0;
+}
+
+// forward over `rows` rows already in X16
+int forward(dacti_mlp *m, int rows) {
+    const uint16_t *a = m->X16;
+    int lda = m->ip[0];
+    for (int l = 0; l < m->L; ++l) {
+        const bool last = l == m->L - 1;
+        int rc = gemm(m, a, lda, 1, m->W16[l], 1, m->ip[l], rows, m->op[l], m->ip[l], m->Z[l], m->op[l], m->A16[l],
+                      m->b[l], last ? 0 : 1);
+        if (rc) return rc;
+        a = m->A16[l];
+        lda = m->op[l];
+    }
+    return 0;
+}
+
+// gradients of the mean loss of nb labelled rows (of `rows`) into gW, gb
+int backward(dacti_mlp *m, int rows, int nb) {
+    const int L = m->L;
+    k_softmax_ce<<<nblk(rows), 256, 0, m->s>>>(m->Z[L - 1], m->op[L - 1], m->n_out, m->yb, rows, 1.0f / (float) nb,
+                                              m->dZ, m->dZ16, m->lossb);
+    MCHK(m, hipGetLastError());
+    for (int l = L - 1; l >= 0; --l) {
+        const uint16_t *aprev = l > 0 ? m->A16[l - 1] : m->X16;
+        const int ldp = m->ip[l], ldz = m->op[l];
+        // gW_l(o, i) = sum_r dZ(r, o) A_prev(r, i)
+        int rc = gemm(m, m->dZ16, 1, ldz, aprev, ldp, 1, m->op[l], m->ip[l], rows, m->gW[l], m->ip[l], nullptr, nullptr, 0);
+        if (rc) return rc;
+        k_colsum<<<nblk(ldz), 256, 0, m->s>>>(m->dZ, ldz, rows, ldz, m->gb[l]);
+        MCHK(m, hipGetLastError());
+        if (l > 0) {
+            // dA(r, i) = sum_o dZ(r, o) W(o, i); then the ReLU mask of layer l-1
+            rc = gemm(m, m->dZ16, ldz, 1, m->W16[l], m->ip[l], 1, rows, m->ip[l], m->op[l], m->dA, m->ip[l], nullptr,
+                      nullptr, 0);
+            if (rc) return rc;
+            const int64_t ne = (int64_t) rows * m->ip[l];
+            k_relu_back<<<nblk(ne), 256, 0, m->s>>>(m->dA, m->Z[l - 1], ne, m->dZ, m->dZ16);
+            MCHK(m, hipGetLastError());
+        }
+    }
+    return 0;
+}
+
+int refresh_bf16(dacti_mlp *m) {
+    for (int l = 0; l < m->L; ++l) {
+        const int64_t n = (int64_t) m->op[l] * m->ip[l];
+        k_to_bf16<<<nblk(n), 256, 0, m->s>>>(m->W[l], m->W16[l], n);
+    }
+    MCHK(m, hipGetLastError());
+    return 0;
+}
+
+int64_t n_params(const dacti_mlp *m) {
+    int64_t n = 0;
+    for (int l = 0; l < m->L; ++l) n += (int64_t) m->orr[l] * m->ir[l] + m->orr[l];
+    return n;
+}
+
+// flat parameters <-> padded device arrays
+int put_params(dacti_mlp *m, const float *p) {
+    for (int l = 0; l < m->L; ++l) {
+        std::vector<float> w((size_t) m->op[l] * m->ip[l], 0.0f), bb((size_t) m->op[l], 0.0f);
+        for (int o = 0; o < m->orr[l]; ++o)
+            for (int i = 0; i < m->ir[l]; ++i) w[(size_t) o * m->ip[l] + i] = *p++;
+        for (int o = 0; o < m->orr[l]; ++o) bb[(size_t) o] = *p++;
+        MCHK(m, hipMemcpy(m->W[l], w.data(), sizeof(float) * w.size(), hipMemcpyHostToDevice));
+        MCHK(m, hipMemcpy(m->b[l], bb.data(), sizeof(float) * bb.size(), hipMemcpyHostToDevice));
+    }
+    return refresh_bf16(m);
+}
+
+int get_padded(dacti_mlp *m, const std::vector<float *> &W, const std::vector<float *> &b, float *p) {
+    MCHK(m, hipStreamSynchronize(m->s));
+    for (int l = 0; l < m->L; ++l) {
+        std::vector<float> w((size_t) m->op[l] * m->ip[l]), bb((size_t) m->op[l]);
+        MCHK(m, hipMemcpy(w.data(), W[l], sizeof(float) * w.size(), hipMemcpyDeviceToHost));
+        MCHK(m, hipMemcpy(bb.data(), b[l], sizeof(float) * bb.size(), hipMemcpyDeviceToHost));
+        for (int o = 0; o < m->orr[l]; ++o)
+            for (int i = 0; i < m->ir[l]; ++i) *p++ = w[(size_t) o * m->ip[l] + i];
+        for (int o = 0; o < m->orr[l]; ++o) *p++ = bb[(size_t) o];
+    }
+    return 0;
+}
+
+void destroy(dacti_mlp *m) {
+    if (!m) return;
+    if (m->s) (void) hipStreamSynchronize(m->s);
+    for (auto *v : {&m->W, &m->b, &m->gW, &m->gb, &m->mW, &m->vW, &m->mb, &m->vb})
+        for (float *p : *v)
+            if (p) (void) hipFree(p);
+    for (auto p : m->W16)
+        if (p) (void) hipFree(p);
+    for (auto p : m->Z)
+        if (p) (void) hipFree(p);
+    for (auto p : m->A16)
+        if (p) (void) hipFree(p);
+    for (void *p : {(void *) m->X16, (void *) m->dZ, (void *) m->dA, (void *) m->dZ16, (void *) m->yb,
+                    (void *) m->lossb})
+        if (p) (void) hipFree(p);
+    if (m->e0) (void) hipEventDestroy(m->e0);
+    if (m->e1) (void) hipEventDestroy(m->e1);
+    if (m->s) (void) hipStreamDestroy(m->s);
+    delete m;
+}
+
+// upload n samples to the device (FP32) and labels
+int upload_data(dacti_mlp *m, const float *x, const int32_t *y, int64_t n, float **dx, int32_t **dy) {
+    *dx = nullptr;
+    *dy = nullptr;
+    MCHK(m, hipMalloc((void **) dx, sizeof(float) * (size_t) std::max<int64_t>(n * m->n_in, 1)));
+    MCHK(m, hipMemcpy(*dx, x, sizeof(float) * (size_t) (n * m->n_in), hipMemcpyHostToDevice));
+    if (y) {
+        MCHK(m, hipMalloc((void **) dy, sizeof(int32_t) * (size_t) std::max<int64_t>(n, 1)));
+        MCHK(m, hipMemcpy(*dy, y, sizeof(int32_t) * (size_t) n, hipMemcpyHostToDevice));
+    }
+    return 0;
+}
+
+// batch rows from the device index array idx (nb entries)
+int load_rows(dacti_mlp *m, const float *dx, const int32_t *dy, const int32_t *idx, int nb, int rows) {
+    k_gather<<<(unsigned) rows, 64, 0, m->s>>>(dx, m->n_in, m->n_in, idx, nb, rows, m->X16, m->ip[0], dy, m->yb);
+    MCHK(m, hipGetLastError());
+    return 0;
+}
+
+// a device copy of host indices (synchronous)
+int upload_idx(dacti_mlp *m, const std::vector<int32_t> &h, int32_t **d) {
+    *d = nullptr;
+    MCHK(m, hipMalloc((void **) d, sizeof(int32_t) * std::max<size_t>(h.size(), 1)));
+    MCHK(m, hipMemcpy(*d, h.data(), sizeof(int32_t) * h.size(), hipMemcpyHostToDevice));
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int dacti_mlp_create(int32_t n_in, int32_t n_hidden, int32_t n_layers, int32_t n_out, uint64_t seed,
+                     dacti_mlp **out) {
+    if (!out || n_in < 1 || n_hidden < 1 || n_layers < 1 || n_out < 2 || n_in > 4096 || n_hidden > 4096 ||
+        n_layers > 16 || n_out > 256) {
+        g_err = "dacti_mlp_create: invalid sizes";
+        return -1;
+    }
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
+        g_err = "dacti_mlp_create: no HIP device available";
+        return -2;
+    }
+    dacti_mlp *m = new dacti_mlp();
+    m->n_in = n_in;
+    m->H = n_hidden;
+    m->NL = n_layers;
+    m->n_out = n_out;
+    m->L = n_layers + 1;
+    for (int l = 0; l < m->L; ++l) {
+        const int in = l == 0 ? n_in : n_hidden, o = l == m->L - 1 ? n_out : n_hidden;
+        m->ir.push_back(in);
+        m->orr.push_back(o);
+        m->ip.push_back(pad16(in));
+        m->op.push_back(pad16(o));
+    }
+    auto fail = [&](int rc) {
+        g_err = m->err;
+        destroy(m);
+        return rc;
+    };
+    if (hipStreamCreateWithFlags(&m->s, hipStreamNonBlocking) != hipSuccess || hipEventCreate(&m->e0) != hipSuccess ||
+        hipEventCreate(&m->e1) != hipSuccess) {
+        m->err = "stream/event creation failed";
+        return fail(-2);
+    }
+    for (auto *v : {&m->W, &m->b, &m->gW, &m->gb, &m->mW, &m->vW, &m->mb, &m->vb}) v->assign(m->L, nullptr);
+    m->W16.assign(m->L, nullptr);
+    for (int l = 0; l < m->L; ++l) {
+        const size_t nw = (size_t) m->op[l] * m->ip[l], nb = (size_t) m->op[l];
+        int rc;
+        if ((rc = dalloc(m, &m->W[l], nw)) || (rc = dalloc(m, &m->gW[l], nw)) || (rc = dalloc(m, &m->mW[l], nw)) ||
+            (rc = dalloc(m, &m->vW[l], nw)) || (rc = dalloc(m, &m->W16[l], nw)) || (rc = dalloc(m, &m->b[l], nb)) ||
+            (rc = dalloc(m, &m->gb[l], nb)) || (rc = dalloc(m, &m->mb[l], nb)) || (rc = dalloc(m, &m->vb[l], nb)))
+            return fail(rc);
+    }
+    std::mt19937_64 rng(seed);
+    std::vector<float> p((size_t) n_params(m));
+    size_t k = 0;
+    for (int l = 0; l < m->L; ++l) {
+        std::uniform_real_distribution<float> u(-1.0f / std::sqrt((float) m->ir[l]), 1.0f / std::sqrt((float) m->ir[l]));
+        for (int i = 0; i < m->orr[l] * m->ir[l]; ++i) p[k++] = u(rng);
+        for (int o = 0; o < m->orr[l]; ++o) p[k++] = 0.0f;
+    }
+    int rc = put_params(m, p.data());
+    if (rc) return fail(rc);
+    if (hipStreamSynchronize(m->s) != hipSuccess) return fail(-2);
+    *out = m;
+    return 0;
+}
+
+int dacti_mlp_n_params(dacti_mlp *m, int64_t *n) {
+    if (!m || !n) return -1;
+    *n = n_params(m);
+    return 0;
+}
+
+int dacti_mlp_get_params(dacti_mlp *m, float *params) {
+    if (!m || !params) return -1;
+    return get_padded(m, m->W, m->b, params);
+}
+
+int dacti_mlp_set_params(dacti_mlp *m, const float *params) {
+    if (!m || !params) return -1;
+    MCHK(m, hipStreamSynchronize(m->s));
+    const int rc = put_params(m, params);
+    if (rc) return rc;
+    MCHK(m, hipStreamSynchronize(m->s));
+    return 0;
+}
+
+int dacti_mlp_forward(dacti_mlp *m, const float *x, int64_t n, float *logits) {
+    if (!m || (!x && n > 0) || (!logits && n > 0) || n < 0) return -1;
+    if (n == 0) return 0;
+    const int chunk = 8192;
+    int rc = ensure_rows(m, (int) std::min<int64_t>(n, chunk));
+    if (rc) return rc;
+    float *dx = nullptr;
+    int32_t *dy = nullptr;
+    rc = upload_data(m, x, nullptr, n, &dx, &dy);
+    if (rc) {
+        if (dx) (void) hipFree(dx);
+        return rc;
+    }
+    std::vector<int32_t> ih((size_t) n);
+    for (int64_t i = 0; i < n; ++i) ih[(size_t) i] = (int32_t) i;
+    int32_t *di = nullptr;
+    rc = upload_idx(m, ih, &di);
+    std::vector<float> z;
+    const int L = m->L;
+    for (int64_t s0 = 0; s0 < n && rc == 0; s0 += chunk) {
+        const int nb = (int) std::min<int64_t>(chunk, n - s0), rows = pad16(nb);
+        if ((rc = load_rows(m, dx, nullptr, di + s0, nb, rows)) || (rc = forward(m, rows))) break;
+        z.resize((size_t) rows * m->op[L - 1]);
+        if (hipMemcpyAsync(z.data(), m->Z[L - 1], sizeof(float) * z.size(), hipMemcpyDeviceToHost, m->s) != hipSuccess ||
+            hipStreamSynchronize(m->s) != hipSuccess) {
+            m->err = "logits copy failed";
+            rc = -2;
+            break;
+        }
+        for (int r = 0; r < nb; ++r)
+            for (int c = 0; c < m->n_out; ++c) logits[(s0 + r) * m->n_out + c] = z[(size_t) r * m->op[L - 1] + c];
+    }
+    (void) hipFree(dx);
+    if (di) (void) hipFree(di);
+    return rc;
+}
+
+int dacti_mlp_loss_grad(dacti_mlp *m, const float *x, const int32_t *y, int64_t n, float *loss, float *grad) {
+    if (!m || !x || !y || n < 1 || n > (1 << 20)) return -1;
+    int rc = ensure_rows(m, (int) n);
+    if (rc) return rc;
+    float *dx = nullptr;
+    int32_t *dy = nullptr;
+    rc = upload_data(m, x, y, n, &dx, &dy);
+    std::vector<int32_t> ih((size_t) n);
+    for (int64_t r = 0; r < n; ++r) ih[(size_t) r] = (int32_t) r;
+    int32_t *di = nullptr;
+    if (!rc) rc = upload_idx(m, ih, &di);
+    const int rows = pad16((int) n);
+    if (!rc) rc = load_rows(m, dx, dy, di, (int) n, rows);
+    if (!rc) rc = forward(m, rows);
+    if (!rc) rc = backward(m, rows, (int) n);
+    if (!rc) {
+        std::vector<float> lb((size_t) rows);
+        if (hipMemcpyAsync(lb.data(), m->lossb, sizeof(float) * lb.size(), hipMemcpyDeviceToHost, m->s) != hipSuccess ||
+            hipStreamSynchronize(m->s) != hipSuccess) {
+            m->err = "loss copy failed";
+            rc = -2;
+        } else if (loss) {
+            double s = 0;
+            for (int64_t r = 0; r < n; ++r) s += lb[(size_t) r];
+            *loss = (float) (s / (double) n);
+        }
+    }
+    if (!rc && grad) rc = get_padded(m, m->gW, m->gb, grad);
+    (void) hipStreamSynchronize(m->s);
+    if (dx) (void) hipFree(dx);
+    if (dy) (void) hipFree(dy);
+    if (di) (void) hipFree(di);
+    return rc;
+}
+
+int dacti_mlp_train(dacti_mlp *m, const float *x, const int32_t *y, int64_t n, int32_t epochs, int32_t batch,
+                    float lr, uint64_t seed, float *loss) {
+    if (!m || !x || !y || n < 1 || n > INT32_MAX / 2 || epochs < 0 || batch < 16 || (batch % 16) != 0 || batch > 65536) {
+        if (m) m->err = "dacti_mlp_train: invalid arguments (batch: a multiple of 16, at most 65536)";
+        return -1;
+    }
+    int rc = ensure_rows(m, batch);
+    if (rc) return rc;
+    float *dx = nullptr, *lall = nullptr;
+    int32_t *dy = nullptr;
+    rc = upload_data(m, x, y, n, &dx, &dy);
+    if (!rc && hipMalloc((void **) &lall, sizeof(float) * (size_t) n) != hipSuccess) {
+        m->err = "loss buffer allocation failed";
+        rc = -3;
+    }
+    std::vector<int32_t> perm((size_t) n);
+    for (int64_t i = 0; i < n; ++i) perm[(size_t) i] = (int32_t) i;
+    int32_t *dperm = nullptr;
+    if (!rc) rc = upload_idx(m, perm, &dperm);
+    std::mt19937_64 rng(seed);
+    m->flops = 0;
+    if (!rc && hipEventRecord(m->e0, m->s) != hipSuccess) rc = -2;
+    float last = 0.0f;
+    for (int ep = 0; ep < epochs && !rc; ++ep) {
+        for (int64_t i = n - 1; i > 0; --i) std::swap(perm[(size_t) i], perm[(size_t) (rng() % (uint64_t) (i + 1))]);
+        // the previous epoch's kernels read dperm: wait, then replace it
+        if (hipStreamSynchronize(m->s) != hipSuccess ||
+            hipMemcpy(dperm, perm.data(), sizeof(int32_t) * (size_t) n, hipMemcpyHostToDevice) != hipSuccess) {
+            m->err = "permutation upload failed";
+            rc = -2;
+            break;
+        }
+        for (int64_t s0 = 0; s0 < n && !rc; s0 += batch) {
+            const int nb = (int) std::min<int64_t>(batch, n - s0), rows = pad16(nb);
+            if ((rc = load_rows(m, dx, dy, dperm + s0, nb, rows)) || (rc = forward(m, rows)) ||
+                (rc = backward(m, rows, nb)))
+                break;
+            k_scatter_loss<<<nblk(nb), 256, 0, m->s>>>(m->lossb, dperm + s0, nb, lall);
+            m->t++;
+            const float c1 = 1.0f - std::pow(0.9f, (float) m->t), c2 = 1.0f - std::pow(0.999f, (float) m->t);
+            for (int l = 0; l < m->L; ++l) {
+                const int64_t nw = (int64_t) m->op[l] * m->ip[l];
+                k_adam<<<nblk(nw), 256, 0, m->s>>>(m->W[l], m->gW[l], m->mW[l], m->vW[l], nw, lr, c1, c2);
+                k_adam<<<nblk(m->op[l]), 256, 0, m->s>>>(m->b[l], m->gb[l], m->mb[l], m->vb[l], m->op[l], lr, c1, c2);
+            }
+            if (hipGetLastError() != hipSuccess) {
+                m->err = "optimizer launch failed";
+                rc = -2;
+                break;
+            }
+            rc = refresh_bf16(m);
+        }
+        if (!rc && ep == epochs - 1) {
+            std::vector<float> la((size_t) n);
+            if (hipMemcpyAsync(la.data(), lall, sizeof(float) * (size_t) n, hipMemcpyDeviceToHost, m->s) != hipSuccess ||
+                hipStreamSynchronize(m->s) != hipSuccess) {
+                m->err = "loss copy failed";
+                rc = -2;
+            } else {
+                double s = 0;
+                for (float v : la) s += v;
+                last = (float) (s / (double) n);
+            }
+        }
+    }
+    if (!rc) {
+        float ms = 0;
+        if (hipEventRecord(m->e1, m->s) != hipSuccess || hipEventSynchronize(m->e1) != hipSuccess ||
+            hipEventElapsedTime(&ms, m->e0, m->e1) != hipSuccess) {
+            m->err = "timing failed";
+            rc = -2;
+        }
+        m->train_ms = ms;
+        m->train_flops = m->flops;
+    }
+    if (loss) *loss = last;
+    (void) hipStreamSynchronize(m->s);
+    if (dx) (void) hipFree(dx);
+    if (dy) (void) hipFree(dy);
+    if (lall) (void) hipFree(lall);
+    if (dperm) (void) hipFree(dperm);
+    return rc;
+}
+
+int dacti_mlp_train_stats(dacti_mlp *m, double *ms, double *flops) {
+    if (!m) return -1;
+    if (ms) *ms = m->train_ms;
+    if (flops) *flops = m->train_flops;
+    return 0;
+}
+
+int dacti_mlp_destroy(dacti_mlp *m) {
+    destroy(m);
+    return 0;
+}
+
+const char *dacti_mlp_last_error(dacti_mlp *m) { return m ? m->err.c_str() : g_err.c_str(); }
+
+}  // extern "C"

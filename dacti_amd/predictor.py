@@ -1,0 +1,130 @@
+"""Learned refinement predictor: data from the solver, features, labels, evaluation.
+
+The solver's own runs are the data: at global step t every active leaf gives
+one sample, its features describe its 3^D neighbourhood now, and its label is
+what the estimator-driven mesh has done at that place h steps later
+(impl/adaptive_refinement.hpp:16-94 decides it in the reference): 0 = the
+leaf holding the cell centre is coarser (coarsened), 1 = the same level
+(kept), 2 = finer (refined).  The network (dacti_amd.mlp, BF16 MFMA on the
+GPU) is trained on these samples; the plain indicator stays the reference
+and judge: the missed-refinement rate is the fraction of the samples the
+mesh refines within h steps that the predictor does not flag.
+
+Features of a leaf (all from its exported node pool):
+  * the primitive state at the 3^D same-size neighbour positions, as
+    differences from the leaf's own state, each scaled by the variable's
+    reference magnitude, and the leaf's own state;
+  * its gradient vector (Navier-Stokes and five-equation models);
+  * its tree level above the base level and log10 of its error estimate.
+"""
+import itertools
+
+import numpy as np
+
+from . import scene
+from .integrator import Integrator
+
+F_LEAF, F_ACTIVE, F_GARBAGE = 1, 2, 4
+
+
+def _active_leaves(nodes):
+    f = nodes["flags"]
+    return np.nonzero(((f & F_LEAF) != 0) & ((f & F_ACTIVE) != 0) & ((f & F_GARBAGE) == 0))[0]
+
+
+def locate(nodes, pts):
+    """Index of the active leaf containing each point (descent from the root,
+    node 1), for points inside the root cell."""
+    idx = np.ones(len(pts), np.int64)
+    fl, cb, ctr = nodes["flags"], nodes["child_base"], nodes["center"]
+    dim = ctr.shape[1]
+    for _ in range(40):
+        inner = (fl[idx] & F_LEAF) == 0
+        if not inner.any():
+            break
+        q = np.zeros(len(pts), np.int64)
+        for d in range(dim):
+            q |= (pts[:, d] > ctr[idx, d]).astype(np.int64) << d
+        idx = np.where(inner, cb[idx] + q, idx)
+    return idx
+
+
+def features(nodes, case):
+    """(leaf ids, float32 features) of the active leaves of one node pool."""
+    lv = _active_leaves(nodes)
+    dim = case.dim
+    p, ctr, size = nodes["p"], nodes["center"], nodes["size"]
+    ne = p.shape[1]
+    scale = np.maximum(np.abs(p[lv]).mean(axis=0), 1e-6)
+    half = np.asarray(case.active_size[:dim]) / 2
+    pc = p[lv]
+    cols = [pc / scale]
+    for off in itertools.product((-1, 0, 1), repeat=dim):
+        if not any(off):
+            continue
+        pts = ctr[lv] + 2 * size[lv, None] * np.asarray(off, float)  # size is the half width
+        inside = np.ones(len(lv), bool)
+        for d in range(dim):
+            if case.periodic[d]:
+                pts[:, d] = (pts[:, d] + half[d]) % (2 * half[d]) - half[d]
+            else:
+                inside &= np.abs(pts[:, d]) < half[d]
+        nb = locate(nodes, np.where(inside[:, None], pts, ctr[lv]))
+        ps = np.where(inside[:, None], p[nb], pc)
+        cols.append((ps - pc) / scale)
+    if case.model != 0:  # Navier-Stokes and five-equation models carry gradients
+        g = nodes["grad"][lv].reshape(len(lv), -1)
+        cols.append(g / np.maximum(np.abs(g).mean(axis=0), 1e-6))
+    cols.append((nodes["tree_level"][lv, None] - case.min_refinement_level).astype(float))
+    cols.append(np.log10(np.maximum(nodes["meta"][lv, 1:2], 1e-12)))
+    return lv, np.concatenate(cols, axis=1).astype(np.float32)
+
+
+def labels(nodes_t, lv, nodes_h):
+    """0 / 1 / 2: the mesh h steps later is coarser / equal / finer at each leaf centre."""
+    at = locate(nodes_h, nodes_t["center"][lv])
+    d = nodes_h["tree_level"][at].astype(int) - nodes_t["tree_level"][lv].astype(int)
+    return (np.sign(d) + 1).astype(np.int32)
+
+
+def indicator_guess(nodes, lv, case):
+    """The method without the network: the estimator's present flags taken as
+    the forecast (refine if the error is above max_error now, coarsen if
+    below min_error, keep otherwise)."""
+    err = nodes["meta"][lv, 1]
+    lo, hi = case.error_threshold
+    return np.where(err > hi, 2, np.where(err < lo, 0, 1)).astype(np.int32)
+
+
+def run_samples(case, steps, horizon, tree=None, log=None):
+    """Step one case on the GPU; returns per-step (features, labels, indicator guess)."""
+    g = Integrator(case, tree if tree is not None else scene.build_tree(case))
+    snaps = [g.get_nodes()]
+    for s in range(steps):
+        _, fb = g.integrate()
+        snaps.append(g.get_nodes())
+        if log:
+            log(f"step {s}")
+        if fb:
+            break
+    g.close()
+    out = []
+    for t in range(len(snaps) - horizon):
+        lv, X = features(snaps[t], case)
+        out.append((X, labels(snaps[t], lv, snaps[t + horizon]), indicator_guess(snaps[t], lv, case)))
+    return out
+
+
+def missed_refinement(pred, y):
+    """Fraction of the samples labelled refine that pred does not flag as refine."""
+    m = y == 2
+    return float((pred[m] != 2).mean()) if m.any() else 0.0
+
+
+class Standardizer:
+    def __init__(self, X):
+        self.mu = X.mean(axis=0)
+        self.sd = np.maximum(X.std(axis=0), 1e-6)
+
+    def __call__(self, X):
+        return ((X - self.mu) / self.sd).astype(np.float32)

@@ -16,8 +16,8 @@ from dacti_amd.case import ConfigError, loads_case
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
-def declared_functions():
-    text = open(os.path.join(ROOT, "include", "dacti_gpu.h")).read()
+def declared_functions(header="dacti_gpu.h"):
+    text = open(os.path.join(ROOT, "include", header)).read()
     return sorted(set(re.findall(r"^\s*(?:const\s+)?\w+\s*\*?\s*(dacti_\w+)\s*\(", text, re.M)))
 
 
@@ -28,6 +28,12 @@ def test_library_exports_every_declared_symbol():
     for n in names:
         assert hasattr(lib, n), n
     assert set(names) == set(_lib.SIGNATURES), "binding signatures out of sync with the header"
+    from dacti_amd import mlp
+    mnames = declared_functions("dacti_mlp.h")
+    assert len(mnames) >= 8
+    for n in mnames:
+        assert hasattr(lib, n), n
+    assert set(mnames) == set(mlp.SIGNATURES), "MLP binding signatures out of sync with the header"
 
 
 def test_struct_layouts_match_header():
