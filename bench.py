@@ -142,6 +142,78 @@ def batched(args):
     print(json.dumps(out), flush=True)
 
 
+def learned(args):
+    """Learned refinement predictor on 2D shock-bubble runs (five-equation
+    model): solver runs generate features and labels (the mesh `horizon`
+    global steps later), the BF16-MFMA MLP is trained on the GPU, and a
+    held-out case is run three ways from the same initial state: remeshing
+    in every substep (the reference's behaviour), remeshing every 4 global
+    steps with the estimator alone, and every `horizon` steps with the
+    predictor refining ahead (estimator as fallback)."""
+    from dacti_amd import predictor, scene, workloads
+    from dacti_amd.mlp import MLP
+
+    def log(msg):
+        print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+    h = args.horizon
+    fam = workloads.shock_bubble_family(args.train_runs + 1, seed=args.seed + 100, base=5, extra_range=(3, 3),
+                                        mach_range=(1.2, 2.2), ratio_range=(0.2, 5.0))
+    Xs, ys, gs = [], [], []
+    t0 = time.perf_counter()
+    for k, (case, prm) in enumerate(fam[:-1]):
+        for X, y, gi in predictor.run_samples(case, args.train_steps, h):
+            Xs.append(X), ys.append(y), gs.append(gi)
+        log(f"training run {k} done ({sum(len(y) for y in ys)} samples, {time.perf_counter() - t0:.0f} s)")
+    X, y = np.concatenate(Xs), np.concatenate(ys)
+    std = predictor.Standardizer(X)
+    m = MLP(X.shape[1], 128, 3, 3, seed=args.seed)
+    loss = m.train(std(X), y, epochs=args.epochs, batch=1024, lr=2e-3, seed=1)
+    ms, fl = m.train_stats()
+    log(f"trained: loss {loss:.3f}, {fl / ms / 1e9:.2f} TFLOP/s")
+    case, prm = fam[-1]
+    tree = scene.build_tree(case)
+    held = predictor.run_samples(case, args.train_steps, h, tree=tree)
+    Xt = np.concatenate([a for a, _, _ in held])
+    yt = np.concatenate([b for _, b, _ in held])
+    gt = np.concatenate([c for _, _, c in held])
+    pred = m.predict(std(Xt))
+    miss_net, miss_ind = predictor.missed_refinement(pred, yt), predictor.missed_refinement(gt, yt)
+    log(f"held-out: missed refinement net {miss_net:.3f}, indicator {miss_ind:.3f}")
+    ref, t_ref, u_ref, _ = predictor.run_cadence(case, tree, args.steps, 1)
+    log(f"reference cadence run: {t_ref:.1f} s")
+    p4, t_p4, u_p4, _ = predictor.run_cadence(case, tree, args.steps, 4)
+    log(f"estimator every 4 steps: {t_p4:.1f} s")
+    ln, t_ln, u_ln, flagged = predictor.run_cadence(case, tree, args.steps, h, model=m, std=std)
+    log(f"learned every {h} steps: {t_ln:.1f} s")
+    out = {
+        "metric": "cell-updates/s (HLLC+MUSCL FP64) & %HBM roofline; learned/batched vs plain",
+        "value": u_ln / t_ln,
+        "unit": "cell-updates/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": 0,
+        "ms_per_step": t_ln * 1e3 / args.steps,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"2D shock-bubble (five-equation model), 32x16 base + 3 levels, learned refinement "
+                               f"every {h} global steps vs the estimator every 4 and every substep",
+                   "held_out_case": prm, "train_runs": args.train_runs, "train_samples": int(len(y)),
+                   "features": int(X.shape[1]), "mlp": "3x128 ReLU, BF16 MFMA, FP32 accumulation",
+                   "horizon_steps": h},
+        "learned": {"cell_updates_per_s": u_ln / t_ln, "seconds": t_ln, "cell_updates": u_ln,
+                    "flagged_leaves": flagged, "rel_l1_vs_every_substep": predictor.relative_l1(ref, ln),
+                    "missed_refinement": miss_net, "accuracy": float((pred == yt).mean())},
+        "plain_every_4": {"cell_updates_per_s": u_p4 / t_p4, "seconds": t_p4, "cell_updates": u_p4,
+                          "rel_l1_vs_every_substep": predictor.relative_l1(ref, p4),
+                          "missed_refinement_of_present_flags": miss_ind},
+        "every_substep": {"cell_updates_per_s": u_ref / t_ref, "seconds": t_ref, "cell_updates": u_ref},
+        "training": {"loss": loss, "ms": ms, "gemm_tflops": fl / ms / 1e9},
+        "roofline": None,
+        "cpu_baseline": None,
+    }
+    m.close()
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -150,8 +222,13 @@ def main():
     ap.add_argument("--level", type=int, default=8)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-probe", action="store_true", help="run a 1 GiB device copy first (counter calibration)")
-    ap.add_argument("--workload", choices=("box", "batched"), default="box",
-                    help="box: 3D periodic Euler box (default); batched: shock-bubble instances, batch vs plain")
+    ap.add_argument("--workload", choices=("box", "batched", "learned"), default="box",
+                    help="box: 3D periodic Euler box (default); batched: shock-bubble instances, batch vs plain; "
+                         "learned: refinement predictor vs the estimator")
+    ap.add_argument("--horizon", type=int, default=16)
+    ap.add_argument("--train-runs", type=int, default=3)
+    ap.add_argument("--train-steps", type=int, default=48)
+    ap.add_argument("--epochs", type=int, default=10)
     ap.add_argument("--instances", type=int, default=64)
     ap.add_argument("--seed", type=int, default=0)
     args = ap.parse_args()
@@ -160,6 +237,9 @@ def main():
         sys.exit(2)
     if args.workload == "batched":
         batched(args)
+        return
+    if args.workload == "learned":
+        learned(args)
         return
 
     from dacti_amd import scene, workloads

@@ -917,7 +917,7 @@ __device__ __forceinline__ void sweep_side(const Dev &d, const MConst &c, int32_
 
 template <int MODEL, int D>
 __device__ __forceinline__ void sweep_body(const Dev &d, const MConst &c, const int32_t *__restrict__ list, int nl, int li, int L,
-                                                double dt, int counter, int pass, Status *st) {
+                                                double dt, int counter, int pass, Status *st, int remesh) {
     using S = Sz<MODEL, D>;
     constexpr int NE = S::NE;
     if (li >= nl) return;
@@ -989,8 +989,12 @@ __device__ __forceinline__ void sweep_body(const Dev &d, const MConst &c, const 
             d.flags[y] = fl;
         }
         uint8_t m = 0;
-        if ((err > c.max_error && (fl & F_CAN_SPLIT)) || (fl & F_SHOULD_SPLIT)) m |= MK_SPLIT;
-        if (err < c.min_error && (fl & F_CAN_UNSPLIT) && ylev > 0 && counter == 1) m |= MK_UNSPLIT_CAND;
+        // with remeshing paused (a cadence set by the host, not a reference
+        // behaviour) every cell takes the LTS decision
+        if (remesh) {
+            if ((err > c.max_error && (fl & F_CAN_SPLIT)) || (fl & F_SHOULD_SPLIT)) m |= MK_SPLIT;
+            if (err < c.min_error && (fl & F_CAN_UNSPLIT) && ylev > 0 && counter == 1) m |= MK_UNSPLIT_CAND;
+        }
         if (m == 0) {
             lts_decide<MODEL, D>(d, c, y, pc, ys, L, dt, counter, lowered);
             m = MK_RA_DONE;
@@ -1541,7 +1545,7 @@ struct Item {
     unsigned long long *hist;  // 16 per-level cluster counts
     int64_t n;                 // pool size
     double dt;
-    int32_t nl, L, counter, signal, flush, pad;
+    int32_t nl, L, counter, signal, flush, remesh;
 };
 constexpr int POOL_TILE = 2048;
 
@@ -1590,7 +1594,7 @@ struct OpSweep {
     int pass;
     __device__ __forceinline__ void operator()(const Item &it, const MConst &c, int li) const {
         if (pass == 1 && it.st->n_deferred == 0) return;
-        sweep_body<MODEL, D>(it.d, c, it.list, it.nl, li, it.L, it.dt, it.counter, pass, it.st);
+        sweep_body<MODEL, D>(it.d, c, it.list, it.nl, li, it.L, it.dt, it.counter, pass, it.st, it.remesh);
     }
 };
 // the interface chain (impl/integrate.hpp:53-58) is skipped for an instance
@@ -1754,6 +1758,17 @@ static __global__ void __launch_bounds__(256) k_pool_flush(const Item *__restric
         const int64_t base = (int64_t) (tile - toff[i]) * POOL_TILE;
         for (int k = 0; k < POOL_TILE; k += 256) flush_node(it.d, base + k + threadIdx.x, it.n, it.L);
     }
+}
+
+// flag listed leaves for splitting at the next remesh (the split keeps the
+// reference's conditions: an active leaf that may still split)
+static __global__ void k_mark_split(Dev d, const int32_t *__restrict__ ids, int64_t n, int64_t n_nodes) {
+    const int64_t i = blockIdx.x * (int64_t) blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int32_t y = ids[i];
+    if (y < 1 || y >= n_nodes) return;
+    const uint8_t fl = d.flags[y];
+    if ((fl & F_LEAF) && (fl & F_ACTIVE) && !(fl & F_GARBAGE) && (fl & F_CAN_SPLIT)) d.flags[y] = fl | F_SHOULD_SPLIT;
 }
 
 static __global__ void k_zero_hist(const Item *__restrict__ items, int n_items) {

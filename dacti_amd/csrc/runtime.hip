@@ -769,6 +769,31 @@ const char *dacti_batch_last_error(dacti_batch *b) {
     return b->err.c_str();
 }
 
+int dacti_set_remesh(dacti_handle *h, int32_t on) {
+    if (!h) return DACTI_E_ARG;
+    h->remesh = on != 0;
+    return DACTI_OK;
+}
+
+int dacti_mark_split(dacti_handle *h, const int32_t *ids, int64_t n) {
+    if (!h || (n > 0 && !ids) || n < 0) return DACTI_E_ARG;
+    if (n == 0) return DACTI_OK;
+    int32_t *d = nullptr;
+    HIPCHK(h, hipMalloc(&d, sizeof(int32_t) * (size_t) n));
+    hipError_t e = hipMemcpy(d, ids, sizeof(int32_t) * (size_t) n, hipMemcpyHostToDevice);
+    if (e == hipSuccess) {
+        k_mark_split<<<blocks(n), 256, 0, h->stream>>>(h->d, d, n, h->n);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    (void) hipFree(d);
+    if (e != hipSuccess) {
+        h->err = std::string("dacti_mark_split: ") + hipGetErrorString(e);
+        return DACTI_E_HIP;
+    }
+    return DACTI_OK;
+}
+
 int dacti_get_stats(dacti_handle *h, dacti_stats *s) {
     if (!h || !s) return DACTI_E_ARG;
     std::memset(s, 0, sizeof *s);

@@ -86,6 +86,7 @@ struct dacti_handle {
     unsigned long long *h_hist = nullptr;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     struct dacti_batch *solo = nullptr;    // batch of this handle alone
+    int remesh = 1;  // 0: refinement marks paused (host-set cadence)
     std::string err;
 };
 
@@ -183,6 +184,7 @@ inline Item base_item(dacti_batch *b, int i) {
     it.st = b->d_st + i;
     it.hist = b->d_hist + 16 * i;
     it.n = h->n;
+    it.remesh = h->remesh;
     return it;
 }
 

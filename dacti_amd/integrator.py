@@ -91,6 +91,17 @@ class Integrator:
         self._check(self.lib.dacti_get_stats(self.h, C.byref(s)), "get_stats")
         return s
 
+    def set_remesh(self, on):
+        """Pause (False) or resume (True) refinement marks; an extension of
+        the reference, which remeshes in every substep."""
+        self._check(self.lib.dacti_set_remesh(self.h, int(bool(on))), "set_remesh")
+
+    def mark_split(self, ids):
+        """Flag node ids to be split at the next remesh (active leaves that may split)."""
+        ids = np.ascontiguousarray(ids, dtype=np.int32)
+        self._check(self.lib.dacti_mark_split(self.h, ids.ctypes.data_as(C.POINTER(C.c_int32)), len(ids)),
+                    "mark_split")
+
     def set_timing(self, on):
         self._check(self.lib.dacti_set_timing(self.h, int(bool(on))), "set_timing")
 

@@ -128,3 +128,47 @@ class Standardizer:
 
     def __call__(self, X):
         return ((X - self.mu) / self.sd).astype(np.float32)
+
+
+def run_cadence(case, tree, steps, every=1, model=None, std=None, log=None):
+    """Step one case with remeshing only in the global steps k with
+    k % every == 0 (every = 1: the reference, which remeshes in every
+    substep).  With a model, before each remeshing step the leaves it
+    forecasts as refined are flagged to split (the estimator still marks as
+    usual: it is the fallback).  Returns (final nodes, seconds, cell updates,
+    flagged leaves)."""
+    import time
+    g = Integrator(case, tree)
+    flagged = 0
+    u0 = g.stats().cell_updates
+    t0 = time.perf_counter()
+    for k in range(steps):
+        on = k % every == 0
+        if on and model is not None:
+            nodes = g.get_nodes()
+            lv, X = features(nodes, case)
+            ids = lv[model.predict(std(X)) == 2]
+            g.mark_split(ids)
+            flagged += len(ids)
+        g.set_remesh(on)
+        _, fb = g.integrate()
+        if log:
+            log(f"step {k}")
+        if fb:
+            raise RuntimeError("NaN state (force_break)")
+    el = time.perf_counter() - t0
+    ups = g.stats().cell_updates - u0
+    nodes = g.get_nodes()
+    g.close()
+    return nodes, el, ups, flagged
+
+
+def relative_l1(ref, other):
+    """Relative L1 difference of the conserved variables of `other` against
+    `ref`, sampled at ref's active leaf centres and weighted by their volume."""
+    lv = _active_leaves(ref)
+    at = locate(other, ref["center"][lv])
+    vol = ref["size"][lv] ** ref["center"].shape[1]
+    num = (np.abs(other["u"][at] - ref["u"][lv]).sum(axis=1) * vol).sum()
+    den = (np.abs(ref["u"][lv]).sum(axis=1) * vol).sum()
+    return float(num / den)

@@ -176,6 +176,17 @@ int dacti_get_stats(dacti_handle *h, dacti_stats *s);
 /* turn HIP-event timing of the per-substep kernels on or off */
 int dacti_set_timing(dacti_handle *h, int32_t on);
 
+/* Remeshing cadence (an extension; the reference remeshes in every substep,
+ * impl/integrate.hpp:59): with on = 0 the sweep sets no split or unsplit
+ * marks, so the mesh stays as it is while the LTS levels still adapt. */
+int dacti_set_remesh(dacti_handle *h, int32_t on);
+
+/* Ask for the listed nodes to be split at the next remesh, as the 2:1
+ * balance does through Node::should_split (integrator/node.hpp); only active
+ * leaves that may still split are flagged.  Used by the learned predictor to
+ * refine ahead of a moving feature. */
+int dacti_mark_split(dacti_handle *h, const int32_t *ids, int64_t n);
+
 int dacti_destroy(dacti_handle *h);
 
 /* message of the last failed call on h (or of the last failed create when

@@ -40,3 +40,31 @@ def test_predictor_forecasts_refinement():
     assert acc > majority
     assert miss_net < miss_ind
     m.close()
+
+
+def test_remesh_pause_and_mark_split():
+    from dacti_amd import scene
+    from dacti_amd.integrator import Integrator
+    # thresholds that never split or coarsen by themselves: only flagged
+    # leaves (and the 2:1 balance they need) split
+    case = workloads.shock_bubble(base=4, extra=2, mach=1.8, error_threshold=(-1.0, 1e300))
+    g = Integrator(case, scene.build_tree(case))
+    g.integrate()
+    a = g.get_nodes()
+    g.set_remesh(False)
+    g.integrate()
+    b = g.get_nodes()
+    # no split or unsplit while paused: same pool and leaf pattern
+    assert len(a["flags"]) == len(b["flags"])
+    np.testing.assert_array_equal(a["flags"] & 1, b["flags"] & 1)
+    lv = predictor._active_leaves(b)
+    cand = lv[(b["tree_level"][lv] < case.max_refinement_level) & ((b["flags"][lv] & 8) != 0)][:20]
+    assert len(cand) == 20
+    g.mark_split(cand)
+    g.set_remesh(True)
+    g.integrate()
+    c = g.get_nodes()
+    split = (c["flags"][cand] & 1) == 0
+    print(f"[predictor] {split.sum()} of {len(cand)} flagged leaves split", flush=True)
+    assert split.all()
+    g.close()
