@@ -67,7 +67,7 @@ struct BOps {
         k_batch<<<gl, b->block, 0, st>>>(it, b->d_mcs, b->d_toff, n, OpApply{});
         k_pool_flush<<<gp, 256, 0, st>>>(it, b->d_toffp, n);
         k_zero_hist<<<(unsigned) n, 64, 0, st>>>(it, n);
-        k_pool_hist<<<gp, 256, 0, st>>>(it, b->d_toffp, n);
+        k_pool_hist<<<std::min(gp, 1024u), 256, 0, st>>>(it, b->d_toffp, n);
         hipError_t e = hipGetLastError();
         if (e == hipSuccess)
             e = hipMemcpyAsync(b->h_hist, b->d_hist, sizeof(unsigned long long) * 16 * (size_t) n,

@@ -884,7 +884,11 @@ __device__ __forceinline__ void sweep_side(const Dev &d, const MConst &c, int32_
         return;
     }
     if (!(d.flags[nb] & F_LEAF)) {
-        interp_p<MODEL, D>(d, nb, pv);
+        // the walk writes through a pointer: give it a temporary, so that pv
+        // itself can stay in registers on the common path
+        double tmp[NE];
+        interp_p<MODEL, D>(d, nb, tmp);
+        for (int e = 0; e < NE; ++e) pv[e] = tmp[e];
         tv = interp_t<D>(d, nb);
         for (int e = 0; e < NE; ++e) qv[e] = pv[e];
     } else {
@@ -896,14 +900,15 @@ __device__ __forceinline__ void sweep_side(const Dev &d, const MConst &c, int32_
             } else {
                 // the coarser leaf's linear reconstruction at the neighbour
                 // position (adaptive_refinement.hpp:48-59)
-                double x[D];
-                for (int a = 0; a < D; ++a) x[a] = ctr[a];
-                x[dim] = ctr[dim] + ys * (2 * DIR - 1.0);
-                if (c.periodic[dim] && DIR == 0 && x[dim] < -0.5 * c.active_size[dim]) x[dim] += c.active_size[dim];
-                if (c.periodic[dim] && DIR == 1 && x[dim] > 0.5 * c.active_size[dim]) x[dim] -= c.active_size[dim];
+                // (dim is a runtime value: select per axis, no indexed
+                // private arrays)
+                double xd = pick<D>(ctr, dim) + ys * (2 * DIR - 1.0);
+                if (c.periodic[dim] && DIR == 0 && xd < -0.5 * c.active_size[dim]) xd += c.active_size[dim];
+                if (c.periodic[dim] && DIR == 1 && xd > 0.5 * c.active_size[dim]) xd -= c.active_size[dim];
                 for (int e = 0; e < NE; ++e) qv[e] = pv[e];
+#pragma unroll
                 for (int a = 0; a < D; ++a) {
-                    const double dx = x[a] - d.center[a * C + nb];
+                    const double dx = (a == dim ? xd : ctr[a]) - d.center[a * C + nb];
                     for (int e = 0; e < NE; ++e) qv[e] = qv[e] + d.slope[(a * NE + e) * C + nb] * dx;
                 }
             }
@@ -1777,27 +1782,48 @@ static __global__ void k_zero_hist(const Item *__restrict__ items, int n_items) 
 }
 
 static __global__ void __launch_bounds__(256) k_pool_hist(const Item *__restrict__ items, const int *__restrict__ toff,
-                                                   int n_items) {
+                                                          int n_items) {
+    // counts gather in LDS over all tiles a block takes and go to the
+    // item's global counters when the item changes (few global atomics)
     __shared__ unsigned int h[16];
     const int total = toff[n_items];
+    int cur = -1;
+    if (threadIdx.x < 16) h[threadIdx.x] = 0;
+    __syncthreads();
     for (int t = blockIdx.x; t < total; t += gridDim.x) {
         const int tile = xcd_block(t, total);
         const int i = find_item(toff, n_items, tile);
+        if (i != cur) {
+            if (cur >= 0 && threadIdx.x < 16 && h[threadIdx.x])
+                atomicAdd(&items[cur].hist[threadIdx.x], (unsigned long long) h[threadIdx.x]);
+            __syncthreads();
+            if (threadIdx.x < 16) h[threadIdx.x] = 0;
+            __syncthreads();
+            cur = i;
+        }
         const Item &it = items[i];
-        if (threadIdx.x < 16) h[threadIdx.x] = 0;
-        __syncthreads();
         const int64_t base = (int64_t) (tile - toff[i]) * POOL_TILE;
         for (int k = 0; k < POOL_TILE; k += 256) {
             const int64_t id = base + k + threadIdx.x;
+            int cl = 16;
             if (id >= 1 && id < it.n) {
-                const int cl = it.d.cluster[id];
-                if (cl >= 0 && cl < 16) atomicAdd(&h[cl], 1u);
+                cl = it.d.cluster[id];
+                if (cl < 0 || cl >= 16) cl = 16;
+            }
+            // the levels present in the wave, then one count per level
+            unsigned present = cl < 16 ? 1u << cl : 0u;
+            for (int o = 32; o > 0; o >>= 1) present |= (unsigned) __shfl_xor((int) present, o);
+            while (present) {
+                const int l = __ffs(present) - 1;
+                present &= present - 1;
+                const uint64_t m = __ballot(cl == l);
+                if ((threadIdx.x & 63) == 0) atomicAdd(&h[l], (unsigned) __popcll(m));
             }
         }
-        __syncthreads();
-        if (threadIdx.x < 16 && h[threadIdx.x]) atomicAdd(&it.hist[threadIdx.x], (unsigned long long) h[threadIdx.x]);
-        __syncthreads();
     }
+    __syncthreads();
+    if (cur >= 0 && threadIdx.x < 16 && h[threadIdx.x])
+        atomicAdd(&items[cur].hist[threadIdx.x], (unsigned long long) h[threadIdx.x]);
 }
 
 }  // namespace dacti

@@ -173,6 +173,17 @@ __device__ inline void flux_F(const MConst &c, const double *p, double up, int d
     }
 }
 
+
+// a[k] for a runtime k < N, as a chain of selects (an indexed private array
+// would be placed in scratch memory)
+template <int N>
+__device__ __forceinline__ double pick(const double *a, int k) {
+    double v = a[0];
+#pragma unroll
+    for (int q = 1; q < N; ++q) v = k == q ? a[q] : v;
+    return v;
+}
+
 // rhs = sum_j A_j(p) slope_j (Hancock time derivative): euler.hpp:86-116,
 // multi.hpp:126-169.  Only the structurally non-zero Jacobian entries are
 // summed, in the reference's (i, j, k) order; the skipped products are
@@ -199,23 +210,26 @@ __device__ inline void rhs_add_dim(const MConst &c, const double *p, const doubl
     using S = Sz<MODEL, D>;
     constexpr int NE = S::NE, IU = S::IU, IP = S::IP;
     const int in = IU + j;
+    // the velocity component j of p and sl, selected without indexing
+    const double p_in = pick<D>(p + IU, j), s_in = pick<D>(sl + IU, j);
+#pragma unroll
     for (int i = 0; i < NE; ++i) {
         double r = rhs[i];
         if (i == 0) {
-            r += p[in] * sl[0];
-            r += p[0] * sl[in];
+            r += p_in * sl[0];
+            r += p[0] * s_in;
         } else if (MODEL == MULTI && i == 1) {
-            r += p[in] * sl[1];
-            r += p[1] * sl[in];
+            r += p_in * sl[1];
+            r += p[1] * s_in;
         } else if (i == in) {
-            r += p[in] * sl[in];
+            r += p_in * s_in;
             r += a_in_ip * sl[IP];
             if constexpr (MODEL == MULTI) r += a_in_ia * sl[S::IA];
         } else if (i == IP) {
-            r += a_ip_in * sl[in];
-            r += p[in] * sl[IP];
+            r += a_ip_in * s_in;
+            r += p_in * sl[IP];
         } else {
-            r += p[in] * sl[i];
+            r += p_in * sl[i];
         }
         rhs[i] = r;
     }
