@@ -24,6 +24,10 @@ sys.path.insert(0, ROOT)
 
 B_ALG = 8 * (2 * 5 + 2)  # bytes per leaf-cell update: state in + out, cell time in + out (euler<3>: NE = 5)
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per leaf-cell update of all substep kernels on this workload, from
+# two rocprofv3 counter runs (FETCH_SIZE x 2.00, WRITE_SIZE x 1.00, calibrated
+# on a 1 GiB device copy): profiles/leaf_update_euler3d_256_hbm_bytes.txt
+MEASURED_BYTES_PER_UPDATE = 1198.0
 
 
 def cpu_baseline(case, tree, seconds_target=20.0):
@@ -298,9 +302,11 @@ def main():
                    "max_cfl": case.max_cfl, "global_dt": s1.global_dt, "max_acti_level": s1.max_acti_level,
                    "cell_updates_timed": updates, "substeps_timed": launches},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "one level substep (flux, update, slope, flag, reassign kernels)",
-                     "bytes_per_update": B_ALG, "avg_launch_ms": avg_ms},
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": MEASURED_BYTES_PER_UPDATE * updates / max(launches, 1),
+                     "kernel": "one level substep (list selection, flux, update, sweep, LTS apply, histogram)",
+                     "bytes_per_update": B_ALG, "measured_bytes_per_update": MEASURED_BYTES_PER_UPDATE,
+                     "avg_launch_ms": avg_ms},
         "cpu_baseline": base,
     }
     print(json.dumps(out), flush=True)
