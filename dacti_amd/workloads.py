@@ -198,3 +198,74 @@ def shock_bubble_family(n, seed=0, base=6, extra_range=(4, 7), mach_range=(1.1, 
         case = shock_bubble(base=base, extra=extra, mach=mach, radius=radius, rho2=ratio)
         out.append((case, {"mach": mach, "radius": radius, "ratio": ratio, "extra": extra}))
     return out
+
+
+SHOCK_DROP_TOML = """
+[scene]
+name = "shock_drop"
+domain_size = [0.016, 0.008, 0.008]
+
+[run]
+print_interval = 1.0
+end_time = 1.0
+
+[integrator]
+min_refinement_level = {base}
+max_refinement_level = {top}
+error_threshold = [{emin}, {emax}]
+error_weights = [1.0e-3, 1.0, 0.0, 0.0, 0.0, 1.0e-5, 1.0]
+max_cfl = 0.5
+
+[model.reference_values]
+rho = [1000.0, 1.2]
+vel = [0.0, 0.0, 0.0]
+p = 1.0e5
+alpha = 0.0
+
+[model.fluid_1]
+gamma = 6.12
+P0 = 3.43e8
+
+[model.fluid_2]
+gamma = 1.4
+P0 = 0.0
+
+[initial_condition]
+type = "jump"
+axis = 0
+location = {shock_x}
+value_neg = [0.0, {r2}, {u2}, 0.0, 0.0, {p2}, 0.0]
+value_pos = [0.0, 1.2, 0.0, 0.0, 0.0, 1.0e5, 0.0]
+
+[[geometry]]
+name = "drop"
+shape = "sphere"
+type = "droplet"
+radius = {radius}
+translation = [{dx}, 0.0, 0.0]
+
+[boundary_conditions.X.neg]
+type = "inflow_r"
+value = [0.0, {r2}, {u2}, 0.0, 0.0, {p2}, 0.0]
+
+[boundary_conditions.X.pos]
+type = "zeroGradient"
+
+[boundary_conditions.Y]
+type = "zeroGradient"
+
+[boundary_conditions.Z]
+type = "zeroGradient"
+"""
+
+
+def shock_drop(base=5, extra=5, mach=2.4, radius=0.001, drop_x=0.0, shock_x=-0.003, error_threshold=(0.02, 0.05)):
+    """3D shock-drop, five-equation stiffened gas in SI units: a Mach `mach`
+    shock in air (gamma 1.4, 1.2 kg/m^3, 1e5 Pa) hits a water drop (gamma
+    6.12, P0 3.43e8 Pa, 1000 kg/m^3) in a 16 x 8 x 8 mm box (fluid 1 = water,
+    alpha = 1 inside the drop)."""
+    r2, u2, p2 = normal_shock(1.2, 1.0e5, mach)
+    toml = SHOCK_DROP_TOML.format(base=base, top=base + extra, emin=repr(float(error_threshold[0])),
+                                  emax=repr(float(error_threshold[1])), shock_x=repr(float(shock_x)), r2=repr(r2),
+                                  u2=repr(u2), p2=repr(p2), radius=repr(float(radius)), dx=repr(float(drop_x)))
+    return loads_case(toml, "multi", 3)

@@ -123,3 +123,14 @@ def amr_sod_case(base=4, top=6, model="euler", emin=0.005, emax=0.02, loc=-0.1):
     if model == "navier_stokes":
         text = text.replace("vel = [0.0, 0.0]", "vel = [0.3, 0.0]")
     return loads_case(text, model, 2)
+
+
+def ns3_box(level=3):
+    """3D periodic box with the viscous Navier-Stokes model (Re 100), the
+    smooth random state of the Euler roofline case."""
+    base = workloads.periodic_box(level=level)
+    toml = workloads.PERIODIC_BOX_TOML.format(level=level).replace(
+        "vel = [0.0, 0.0, 0.0]", "vel = [0.0, 0.0, 0.0]\nRe = 100.0\nL = 1.0\nPr = 0.72")
+    case = loads_case(toml, "navier_stokes", 3)
+    case.ic_grid = base.ic_grid
+    return case

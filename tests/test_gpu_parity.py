@@ -75,3 +75,14 @@ def test_parity_multi2d_deep_subtrees():
     # interpolation walk must follow like the reference's recursion
     case = workloads.shock_bubble(base=4, extra=5, mach=2.0, radius=0.2, rho2=0.5)
     _run_pair(case, 2, "multi2d shock-bubble 16x8+5")
+
+
+def test_parity_multi3d_shock_drop():
+    # stiffened-gas water drop (P0 3.43e8 Pa) in air, SI units, AMR + LTS + interface chain in 3D
+    case = workloads.shock_drop(base=3, extra=2, radius=0.002, shock_x=-0.005)
+    _run_pair(case, 2, "multi3d shock-drop 8x4x4+2")
+
+
+def test_parity_navier_stokes_3d_box():
+    from tests.helpers import ns3_box
+    _run_pair(ns3_box(level=3), 2, "navier-stokes 3d 8^3")
