@@ -29,9 +29,9 @@ struct BOps {
         const unsigned gl = list_grid(b), gp = pool_grid(b);
         const int B = b->block, n = b->n;
         Item *it = b->d_items;
-        k_sel_count<<<gp, 256, 0, st>>>(it, b->d_toffp, n, b->d_tilecnt);
-        k_sel_scan<<<1, 1024, 0, st>>>(it, b->d_toffp, n, b->d_tilecnt);
-        k_sel_scatter<<<gp, 256, 0, st>>>(it, b->d_toffp, n, b->d_tilecnt);
+        k_sel_count<0><<<gp, 256, 0, st>>>(it, b->d_toffp, n, b->d_tilecnt);
+        k_sel_scan<0><<<1, 1024, 0, st>>>(it, b->d_toffp, n, b->d_tilecnt);
+        k_sel_scatter<0><<<gp, 256, 0, st>>>(it, b->d_toffp, n, b->d_tilecnt);
         k_batch<<<gl, B, 0, st>>>(it, b->d_mcs, b->d_toff, n, OpFluxSimple<MODEL, D>{});
         k_batch<<<gl, B, 0, st>>>(it, b->d_mcs, b->d_toff, n, OpFluxRest<MODEL, D>{});
         k_batch<<<gl, B, 0, st>>>(it, b->d_mcs, b->d_toff, n, OpUpdate<MODEL, D>{});
@@ -129,9 +129,12 @@ struct BOps {
         for (int i = 0; i < b->n; ++i) {
             dacti_handle *h = b->hs[(size_t) i];
             b->h_items[i].d = h->d;
+            b->h_items[i].d.err = &b->d_st[i].dev_error;
             b->h_items[i].n = h->n;
             b->h_items[i].list = h->d_list;
             b->h_items[i].rest = h->d_list2;
+            b->h_items[i].rn = h->d_rn;
+            b->h_items[i].off = h->d_off;
             b->h_toffp[i] = (int) tp;
             if (changed[(size_t) i]) tp += (h->n + POOL_TILE - 1) / POOL_TILE;
         }
@@ -145,6 +148,10 @@ struct BOps {
             e = hipMemcpyAsync(b->d_toff, b->h_toff, sizeof(int) * n1, hipMemcpyHostToDevice, b->stream);
         if (e == hipSuccess)
             e = hipMemcpyAsync(b->d_toffp, b->h_toffp, sizeof(int) * n1, hipMemcpyHostToDevice, b->stream);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(b->d_toffr, b->h_toffr, sizeof(int) * n1, hipMemcpyHostToDevice, b->stream);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(b->d_toff2, b->h_toff2, sizeof(int) * n1, hipMemcpyHostToDevice, b->stream);
         if (e != hipSuccess) {
             b->err = std::string("table upload: ") + hipGetErrorString(e);
             return DACTI_E_HIP;
@@ -161,81 +168,90 @@ struct BOps {
         return sync(b);
     }
 
+    static unsigned grid_for(const dacti_batch *b, int64_t tiles) {
+        return b->grid > 0 ? (unsigned) b->grid : (unsigned) std::max<int64_t>(tiles, 1);
+    }
+
     // Split / unsplit application and neighbour update
     // (impl/adaptive_refinement.hpp:97-262), then the LTS decision of the
-    // cells that were marked or created, for every instance in ids.  Each
-    // phase is launched for all instances before the one wait that its
-    // decisions need.
-    static int remesh_all(dacti_batch *b, int L, const std::vector<int> &ids) {
+    // cells that were marked or created, for every instance in ids, each
+    // phase one launch over all of them.  One wait: the split counts decide
+    // the pool growth.
+    static int remesh_all(dacti_batch *b, int L, const std::vector<int> &ids, const std::vector<char> &changed) {
         const hipStream_t st = b->stream;
-        for (int i : ids) {
-            dacti_handle *h = b->hs[(size_t) i];
-            const int nl = b->h_items[i].nl;
-            k_mark2<D><<<blocks(nl), 256, 0, st>>>(h->d, h->d_list, nl, L, h->d_rn, b->d_st + i);
+        const int n = b->n, B = b->block;
+        std::vector<char> meshing((size_t) n, 0);
+        for (int i : ids) meshing[(size_t) i] = 1;
+        int64_t tr = 0;
+        for (int i = 0; i < n; ++i) {
+            b->h_items[i].meshing = meshing[(size_t) i];
+            b->h_toffr[i] = (int) tr;
+            if (meshing[(size_t) i]) tr += (b->h_items[i].nl + 255) / 256;
+            b->h_toff2[i] = 0;
         }
+        b->h_toffr[n] = (int) tr;
+        b->h_toff2[n] = 0;
+        int rc = upload_tables(b, changed);
+        if (rc) return rc;
+        k_batch<<<list_grid(b), B, 0, st>>>(b->d_items, b->d_mcs, b->d_toff, n, OpMark2<D>{});
         if (hipGetLastError() != hipSuccess) {
             b->err = "remesh launch failed";
             return DACTI_E_HIP;
         }
-        int rc = copy_status(b);
-        if (rc) return rc;
-        std::vector<char> topo((size_t) b->n, 0);
-        for (int i : ids) {
-            Bind bind(b, i);
-            dacti_handle *h = bind.h;
-            const int nl = b->h_items[i].nl;
-            const int n_ready = b->h_st[i].n_ready_split;
-            const int n_new = b->h_st[i].n_new_blocks;
-            if (n_ready > 0) {
-                size_t bytes = h->tmp_bytes;
-                HIPCHK(b, hipcub::DeviceScan::ExclusiveSum(h->d_tmp, bytes, h->d_rn, h->d_off, nl, st));
-                const int64_t need = h->n + (int64_t) S::NCH * n_new;
-                if (need > h->d.C) {
-                    // keep the list across the reallocation
-                    std::vector<int32_t> keep((size_t) nl), offs((size_t) nl);
-                    HIPCHK(b, hipMemcpyAsync(keep.data(), h->d_list, sizeof(int32_t) * nl, hipMemcpyDeviceToHost, st));
-                    HIPCHK(b, hipMemcpyAsync(offs.data(), h->d_off, sizeof(int32_t) * nl, hipMemcpyDeviceToHost, st));
-                    HIPCHK(b, hipStreamSynchronize(st));
-                    rc = ensure_capacity(h, need + need / 4);
-                    if (rc) {
-                        fail_from(b, i);
-                        return rc;
-                    }
-                    HIPCHK(b, hipMemcpyAsync(h->d_list, keep.data(), sizeof(int32_t) * nl, hipMemcpyHostToDevice, st));
-                    HIPCHK(b, hipMemcpyAsync(h->d_off, offs.data(), sizeof(int32_t) * nl, hipMemcpyHostToDevice, st));
-                    HIPCHK(b, hipStreamSynchronize(st));
-                }
-                k_split_slopes<MODEL, D><<<blocks(nl), 256, 0, st>>>(h->d, h->mc, h->d_list, nl);
-                k_split<MODEL, D><<<blocks(nl), 256, 0, st>>>(h->d, h->mc, h->d_list, nl, h->d_off, h->n, L);
-                h->n += (int64_t) S::NCH * n_new;
-                topo[(size_t) i] = 1;
-            }
-            k_unsplit<MODEL, D><<<blocks(nl), 256, 0, st>>>(h->d, h->mc, h->d_list, nl, L, b->d_st + i);
-            HIPCHK(b, hipGetLastError());
-        }
         rc = copy_status(b);
         if (rc) return rc;
-        // cluster L now: the listed cells (split parents and coarsened
-        // children still in it, to be dropped), new children, unsplit parents
-        for (int i : ids) {
+        // pool growth on the host (rare reallocation), pool size before the splits
+        int64_t t2 = 0;
+        for (int i = 0; i < n; ++i) {
+            b->h_toff2[i] = (int) t2;
+            if (!meshing[(size_t) i]) continue;
             dacti_handle *h = b->hs[(size_t) i];
-            if (topo[(size_t) i] || b->h_st[i].n_unsplit > 0)
-                k_neighbors<D><<<blocks(h->n), 256, 0, st>>>(h->d, h->mc, h->n);
-            hipcub::CountingInputIterator<int32_t> it(1);
-            size_t bytes = h->tmp_bytes;
-            HIPCHK(b, hipcub::DeviceSelect::If(h->d_tmp, bytes, it, h->d_list2, h->d_nsel, (int) (h->n - 1),
-                                               ClusterIs{h->d.cluster, L}, st));
-            HIPCHK(b, hipMemcpyAsync(h->h_nsel, h->d_nsel, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+            const int nl = b->h_items[i].nl;
+            const int n_ready = b->h_st[i].n_ready_split, n_new = b->h_st[i].n_new_blocks;
+            const int64_t need = h->n + (int64_t) S::NCH * n_new;
+            if (need > h->d.C) {
+                // keep the list and the new-block flags across the reallocation
+                Bind bind(b, i);
+                std::vector<int32_t> keep((size_t) nl);
+                std::vector<uint8_t> rn((size_t) nl);
+                HIPCHK(b, hipMemcpyAsync(keep.data(), h->d_list, sizeof(int32_t) * nl, hipMemcpyDeviceToHost, st));
+                HIPCHK(b, hipMemcpyAsync(rn.data(), h->d_rn, (size_t) nl, hipMemcpyDeviceToHost, st));
+                HIPCHK(b, hipStreamSynchronize(st));
+                rc = ensure_capacity(h, need + need / 4);
+                if (rc) {
+                    fail_from(b, i);
+                    return rc;
+                }
+                HIPCHK(b, hipMemcpyAsync(h->d_list, keep.data(), sizeof(int32_t) * nl, hipMemcpyHostToDevice, st));
+                HIPCHK(b, hipMemcpyAsync(h->d_rn, rn.data(), (size_t) nl, hipMemcpyHostToDevice, st));
+                HIPCHK(b, hipStreamSynchronize(st));
+            }
+            b->h_items[i].n_before = h->n;
+            h->n += (int64_t) S::NCH * n_new;
+            // the rebuilt list: the listed cells, the new children and at
+            // most one unsplit parent per listed cell
+            t2 += (2 * (int64_t) nl + (int64_t) S::NCH * n_ready + B - 1) / B;
         }
-        rc = sync(b);
+        b->h_toff2[n] = (int) t2;
+        rc = batch_ensure_tiles(b, std::max<int64_t>(tr, 1));
         if (rc) return rc;
-        for (int i : ids) {
-            dacti_handle *h = b->hs[(size_t) i];
-            const int n2 = *h->h_nsel;
-            if (n2 > 0)
-                k_reassign<MODEL, D><<<blocks(n2), 256, 0, st>>>(h->d, h->mc, h->d_list2, n2, L, b->h_items[i].dt,
-                                                                 h->acti_counter[L], b->d_st + i);
-        }
+        rc = upload_tables(b, changed);  // new pool sizes and pool tiles
+        if (rc) return rc;
+        // new-block offsets, splits, unsplits
+        const unsigned gr = grid_for(b, tr);
+        k_rn_count<<<gr, 256, 0, st>>>(b->d_items, b->d_toffr, n, b->d_tilecnt);
+        k_sel_scan<2><<<1, 1024, 0, st>>>(b->d_items, b->d_toffr, n, b->d_tilecnt);
+        k_rn_scatter<<<gr, 256, 0, st>>>(b->d_items, b->d_toffr, n, b->d_tilecnt);
+        k_batch<<<list_grid(b), B, 0, st>>>(b->d_items, b->d_mcs, b->d_toff, n, OpSplitSlopes<MODEL, D>{});
+        k_batch<<<list_grid(b), B, 0, st>>>(b->d_items, b->d_mcs, b->d_toff, n, OpSplit<MODEL, D>{});
+        k_batch<<<list_grid(b), B, 0, st>>>(b->d_items, b->d_mcs, b->d_toff, n, OpUnsplit<MODEL, D>{});
+        // neighbours, the rebuilt cluster list, its LTS decisions
+        const unsigned gp = pool_grid(b);
+        k_pool_neighbors<D><<<gp, 256, 0, st>>>(b->d_items, b->d_mcs, b->d_toffp, n);
+        k_sel_count<1><<<gp, 256, 0, st>>>(b->d_items, b->d_toffp, n, b->d_tilecnt);
+        k_sel_scan<1><<<1, 1024, 0, st>>>(b->d_items, b->d_toffp, n, b->d_tilecnt);
+        k_sel_scatter<1><<<gp, 256, 0, st>>>(b->d_items, b->d_toffp, n, b->d_tilecnt);
+        k_batch<<<grid_for(b, t2), B, 0, st>>>(b->d_items, b->d_mcs, b->d_toff2, n, OpReassign<MODEL, D>{});
         HIPCHK(b, hipGetLastError());
         return DACTI_OK;
     }
@@ -332,7 +348,7 @@ struct BOps {
             for (int i = 0; i < n; ++i)
                 if (b->h_items[i].nl > 0 && b->h_st[i].n_marked > 0) marked.push_back(i);
             if (!marked.empty()) {
-                rc = remesh_all(b, L, marked);
+                rc = remesh_all(b, L, marked, changed);
                 if (rc) return rc;
                 // pools may have grown or moved: new items and pool tiles
                 rc = sync(b);

@@ -53,6 +53,7 @@ struct Status {
     int32_t dropped;
     int32_t raised[16];
     int32_t n_list;         // size of the substep's cluster list as selected on the device
+    int32_t n_list2;        // size of the cluster list rebuilt after a remesh
 };
 
 __device__ __forceinline__ bool is_leaf(const Dev &d, int32_t i) { return d.flags[i] & F_LEAF; }
@@ -1179,9 +1180,8 @@ __device__ __forceinline__ void smooth_curvature_body(const Dev &d, const MConst
 // --------------------------------------------------- refinement flagging
 // unsplit detection (:115-149) and split readiness (:154-165)
 template <int D>
-__global__ void k_mark2(Dev d, const int32_t *__restrict__ list, int nl, int L, uint8_t *ready_new,
-                        Status *st) {
-    const int li = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void mark2_body(const Dev &d, const int32_t *__restrict__ list, int nl, int li, int L,
+                                           uint8_t *ready_new, Status *st) {
     if (li >= nl) return;
     const int64_t C = d.C;
     const int32_t y = list[li];
@@ -1215,8 +1215,8 @@ __global__ void k_mark2(Dev d, const int32_t *__restrict__ list, int nl, int L, 
 
 // slopes of the cells about to split, on the pre-split tree (:169)
 template <int MODEL, int D>
-__global__ void k_split_slopes(Dev d, MConst c, const int32_t *__restrict__ list, int nl) {
-    const int li = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void split_slopes_body(const Dev &d, const MConst &c, const int32_t *__restrict__ list,
+                                                  int nl, int li) {
     if (li >= nl) return;
     const int32_t y = list[li];
     if (d.mark[y] & MK_READY) compute_slope_dev<MODEL, D>(d, c, y);
@@ -1225,11 +1225,10 @@ __global__ void k_split_slopes(Dev d, MConst c, const int32_t *__restrict__ list
 // Domain::split (domain.hpp:192-247) + child set-up (:174-195).  new_off is
 // the exclusive scan of "needs a new child block" over the list.
 template <int MODEL, int D>
-__global__ void k_split(Dev d, MConst c, const int32_t *__restrict__ list, int nl, const int32_t *new_off,
-                        int64_t n_before, int L) {
+__device__ __forceinline__ void split_body(const Dev &d, const MConst &c, const int32_t *__restrict__ list, int nl,
+                                           int li, const int32_t *new_off, int64_t n_before, int L) {
     using S = Sz<MODEL, D>;
     constexpr int NE = S::NE;
-    const int li = blockIdx.x * blockDim.x + threadIdx.x;
     if (li >= nl) return;
     const int64_t C = d.C;
     const int32_t y = list[li];
@@ -1299,10 +1298,10 @@ __device__ void interpolated_cell_acc(const Dev &d, int32_t idx, double &t, doub
                                       double *ts, const MConst &c);
 
 template <int MODEL, int D>
-__global__ void k_unsplit(Dev d, MConst c, const int32_t *__restrict__ list, int nl, int L, Status *st) {
+__device__ __forceinline__ void unsplit_body(const Dev &d, const MConst &c, const int32_t *__restrict__ list, int nl,
+                                             int li, int L, Status *st) {
     using S = Sz<MODEL, D>;
     constexpr int NE = S::NE, NCH = S::NCH;
-    const int li = blockIdx.x * blockDim.x + threadIdx.x;
     if (li >= nl) return;
     const int64_t C = d.C;
     const int32_t y = list[li];
@@ -1406,8 +1405,7 @@ static __global__ void k_clear_marks(Dev d, const int32_t *__restrict__ list, in
 
 // update_neighbors (domain.hpp:249-293) for every active leaf
 template <int D>
-__global__ void __launch_bounds__(256) k_neighbors(Dev d, MConst c, int64_t n) {
-    const int64_t i = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void neighbors_node(const Dev &d, const MConst &c, int64_t i, int64_t n) {
     if (i < 1 || i >= n) return;
     const uint8_t fl = d.flags[i];
     if (!(fl & F_LEAF) || !(fl & F_ACTIVE) || (fl & F_GARBAGE)) return;
@@ -1428,13 +1426,17 @@ __global__ void __launch_bounds__(256) k_neighbors(Dev d, MConst c, int64_t n) {
         }
 }
 
+template <int D>
+__global__ void __launch_bounds__(256) k_neighbors(Dev d, MConst c, int64_t n) {
+    neighbors_node<D>(d, c, (int64_t) blockIdx.x * blockDim.x + threadIdx.x, n);
+}
+
 // ------------------------------------------------------- LTS reassignment
 // impl/adaptive_refinement.hpp:264-322
 template <int MODEL, int D>
-__global__ void __launch_bounds__(256) k_reassign(Dev d, MConst c, const int32_t *__restrict__ list, int nl, int L,
-                                                   double dt, int counter, Status *st) {
+__device__ __forceinline__ void reassign_body(const Dev &d, const MConst &c, const int32_t *__restrict__ list, int nl,
+                                              int li0, int L, double dt, int counter, Status *st) {
     using S = Sz<MODEL, D>;
-    const int li0 = blockIdx.x * blockDim.x + threadIdx.x;
     if (li0 - (int) (threadIdx.x & 63) >= nl) return;  // whole wavefront past the end
     const bool live = li0 < nl;
     const int li = live ? li0 : nl - 1;
@@ -1551,6 +1553,12 @@ struct Item {
     int64_t n;                 // pool size
     double dt;
     int32_t nl, L, counter, signal, flush, remesh;
+    // remesh of this substep: ready/new-block flags and new-block offsets per
+    // list entry, pool size before the splits, whether this instance remeshes
+    uint8_t *rn;
+    int32_t *off;
+    int64_t n_before;
+    int32_t meshing, pad2;
 };
 constexpr int POOL_TILE = 2048;
 
@@ -1674,7 +1682,8 @@ __device__ __forceinline__ int block_exclusive_scan_1024(int v, int *lds, int &t
 // Cluster-L lists of the batch, in node order (the order of the reference's
 // acti_clusters after a rebuild is irrelevant to the results; node order keeps
 // the sweeps coalesced): per pool tile a count, one scan per item, a scatter.
-static __global__ void __launch_bounds__(256) k_sel_count(const Item *__restrict__ items, const int *__restrict__ toff,
+template <int OUT>
+__global__ void __launch_bounds__(256) k_sel_count(const Item *__restrict__ items, const int *__restrict__ toff,
                                                    int n_items, int *tilecnt) {
     __shared__ int wc[4];
     const int total = toff[n_items];
@@ -1683,7 +1692,7 @@ static __global__ void __launch_bounds__(256) k_sel_count(const Item *__restrict
         const int i = find_item(toff, n_items, tile);
         const Item &it = items[i];
         int cnt = 0;
-        if (it.nl > 0) {
+        if (OUT == 0 ? it.nl > 0 : it.meshing != 0) {
             const int64_t base = (int64_t) (tile - toff[i]) * POOL_TILE;
             for (int k = 0; k < POOL_TILE; k += 256) {
                 const int64_t id = base + k + threadIdx.x;
@@ -1698,9 +1707,11 @@ static __global__ void __launch_bounds__(256) k_sel_count(const Item *__restrict
     }
 }
 
-// one block of 1024 threads: tile offsets within each item, the item's list
-// size, and a cleared status
-static __global__ void __launch_bounds__(1024) k_sel_scan(const Item *__restrict__ items, const int *__restrict__ toff,
+// one block of 1024 threads: tile offsets within each item and the item's
+// list size (OUT 0: the substep's list, with a cleared status; OUT 1: the
+// list rebuilt after a remesh; OUT 2: offsets only)
+template <int OUT>
+__global__ void __launch_bounds__(1024) k_sel_scan(const Item *__restrict__ items, const int *__restrict__ toff,
                                                    int n_items, int *tilecnt) {
     __shared__ int lds[32];
     for (int i = 0; i < n_items; ++i) {
@@ -1716,13 +1727,18 @@ static __global__ void __launch_bounds__(1024) k_sel_scan(const Item *__restrict
         }
         if (threadIdx.x == 0) {
             Status *st = items[i].st;
-            *st = Status{};
-            st->n_list = running;
+            if (OUT == 0) {
+                *st = Status{};
+                st->n_list = running;
+            } else if (OUT == 1) {
+                st->n_list2 = running;
+            }
         }
     }
 }
 
-static __global__ void __launch_bounds__(256) k_sel_scatter(const Item *__restrict__ items, const int *__restrict__ toff,
+template <int OUT>
+__global__ void __launch_bounds__(256) k_sel_scatter(const Item *__restrict__ items, const int *__restrict__ toff,
                                                      int n_items, const int *__restrict__ tileoff) {
     __shared__ int wc[4];
     const int total = toff[n_items];
@@ -1732,7 +1748,8 @@ static __global__ void __launch_bounds__(256) k_sel_scatter(const Item *__restri
         const int tile = xcd_block(t, total);
         const int i = find_item(toff, n_items, tile);
         const Item &it = items[i];
-        if (it.nl == 0) continue;  // uniform over the block
+        if (OUT == 0 ? it.nl == 0 : it.meshing == 0) continue;  // uniform over the block
+        int32_t *out = OUT == 0 ? it.list : it.rest;
         const int64_t base = (int64_t) (tile - toff[i]) * POOL_TILE;
         int pos = tileoff[tile];
         for (int k = 0; k < POOL_TILE; k += 256) {
@@ -1743,7 +1760,7 @@ static __global__ void __launch_bounds__(256) k_sel_scatter(const Item *__restri
             __syncthreads();
             int before = 0;
             for (int q = 0; q < w; ++q) before += wc[q];
-            if (pr) it.list[pos + before + __popcll(m & lt)] = (int32_t) id;
+            if (pr) out[pos + before + __popcll(m & lt)] = (int32_t) id;
             pos += wc[0] + wc[1] + wc[2] + wc[3];
             __syncthreads();
         }
@@ -1794,6 +1811,7 @@ static __global__ void __launch_bounds__(256) k_pool_hist(const Item *__restrict
         const int tile = xcd_block(t, total);
         const int i = find_item(toff, n_items, tile);
         if (i != cur) {
+            __syncthreads();  // every wave has counted the previous tile
             if (cur >= 0 && threadIdx.x < 16 && h[threadIdx.x])
                 atomicAdd(&items[cur].hist[threadIdx.x], (unsigned long long) h[threadIdx.x]);
             __syncthreads();
@@ -1824,6 +1842,103 @@ static __global__ void __launch_bounds__(256) k_pool_hist(const Item *__restrict
     __syncthreads();
     if (cur >= 0 && threadIdx.x < 16 && h[threadIdx.x])
         atomicAdd(&items[cur].hist[threadIdx.x], (unsigned long long) h[threadIdx.x]);
+}
+
+// ------------------------------------------------- batched remesh phases
+// (impl/adaptive_refinement.hpp:97-322) for the instances with marks
+// (Item::meshing); the others return at once.
+template <int D>
+struct OpMark2 {
+    __device__ __forceinline__ void operator()(const Item &it, const MConst &, int li) const {
+        if (!it.meshing) return;
+        mark2_body<D>(it.d, it.list, it.nl, li, it.L, it.rn, it.st);
+    }
+};
+
+// exclusive scan of the new-block flags over each item's list: per list
+// tile a count (tiles of 256 entries), one scan per item, then the offsets
+static __global__ void __launch_bounds__(256) k_rn_count(const Item *__restrict__ items, const int *__restrict__ toff,
+                                                  int n_items, int *tilecnt) {
+    __shared__ int wc[4];
+    const int total = toff[n_items];
+    for (int t = blockIdx.x; t < total; t += gridDim.x) {
+        const int i = find_item(toff, n_items, t);
+        const Item &it = items[i];
+        const int li = (t - toff[i]) * 256 + (int) threadIdx.x;
+        const bool pr = it.meshing && li < it.nl && it.rn[li];
+        const int c = __popcll(__ballot(pr));
+        if ((threadIdx.x & 63) == 0) wc[threadIdx.x >> 6] = c;
+        __syncthreads();
+        if (threadIdx.x == 0) tilecnt[t] = wc[0] + wc[1] + wc[2] + wc[3];
+        __syncthreads();
+    }
+}
+
+static __global__ void __launch_bounds__(256) k_rn_scatter(const Item *__restrict__ items, const int *__restrict__ toff,
+                                                    int n_items, const int *__restrict__ tileoff) {
+    __shared__ int wc[4];
+    const int total = toff[n_items];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const uint64_t lt = lane ? (~0ull >> (64 - lane)) : 0ull;
+    for (int t = blockIdx.x; t < total; t += gridDim.x) {
+        const int i = find_item(toff, n_items, t);
+        const Item &it = items[i];
+        const int li = (t - toff[i]) * 256 + (int) threadIdx.x;
+        const bool pr = it.meshing && li < it.nl && it.rn[li];
+        const uint64_t m = __ballot(pr);
+        if (lane == 0) wc[w] = __popcll(m);
+        __syncthreads();
+        int before = 0;
+        for (int q = 0; q < w; ++q) before += wc[q];
+        if (it.meshing && li < it.nl) it.off[li] = tileoff[t] + before + __popcll(m & lt);
+        __syncthreads();
+    }
+}
+
+template <int MODEL, int D>
+struct OpSplitSlopes {
+    __device__ __forceinline__ void operator()(const Item &it, const MConst &c, int li) const {
+        if (!it.meshing) return;
+        split_slopes_body<MODEL, D>(it.d, c, it.list, it.nl, li);
+    }
+};
+template <int MODEL, int D>
+struct OpSplit {
+    __device__ __forceinline__ void operator()(const Item &it, const MConst &c, int li) const {
+        if (!it.meshing) return;
+        split_body<MODEL, D>(it.d, c, it.list, it.nl, li, it.off, it.n_before, it.L);
+    }
+};
+template <int MODEL, int D>
+struct OpUnsplit {
+    __device__ __forceinline__ void operator()(const Item &it, const MConst &c, int li) const {
+        if (!it.meshing) return;
+        unsplit_body<MODEL, D>(it.d, c, it.list, it.nl, li, it.L, it.st);
+    }
+};
+// LTS decision of the cells of the rebuilt list (size on the device)
+template <int MODEL, int D>
+struct OpReassign {
+    __device__ __forceinline__ void operator()(const Item &it, const MConst &c, int li) const {
+        if (!it.meshing) return;
+        reassign_body<MODEL, D>(it.d, c, it.rest, it.st->n_list2, li, it.L, it.dt, it.counter, it.st);
+    }
+};
+
+template <int D>
+__global__ void __launch_bounds__(256) k_pool_neighbors(const Item *__restrict__ items,
+                                                        const MConst *__restrict__ mcs, const int *__restrict__ toff,
+                                                        int n_items) {
+    const int total = toff[n_items];
+    for (int t = blockIdx.x; t < total; t += gridDim.x) {
+        const int tile = xcd_block(t, total);
+        const int i = find_item(toff, n_items, tile);
+        const Item &it = items[i];
+        // only where the topology changed (a split or an unsplit happened)
+        if (!it.meshing || (it.st->n_ready_split == 0 && it.st->n_unsplit == 0)) continue;
+        const int64_t base = (int64_t) (tile - toff[i]) * POOL_TILE;
+        for (int k = 0; k < POOL_TILE; k += 256) neighbors_node<D>(it.d, mcs[i], base + k + threadIdx.x, it.n);
+    }
 }
 
 }  // namespace dacti

@@ -97,7 +97,7 @@ int alloc_pool(dacti_handle *h, int64_t C, Dev *old, int64_t n) {
     HIPCHK(h, hipcub::DeviceReduce::Max(nullptr, b3, nd.scratch, (double *) nd.tot, (int) C, h->stream));
     h->tmp_bytes = std::max(b1, std::max(b2, b3)) + 1024;
     HIPCHK(h, hipMalloc(&h->d_tmp, h->tmp_bytes));
-    nd.err = &h->d_status->dev_error;
+    nd.err = &h->d_status_home->dev_error;
     h->d = nd;
     return DACTI_OK;
 }
@@ -216,7 +216,7 @@ int gc(dacti_handle *h) {
     HIPCHK(h, hipMemsetAsync(nd.cluster + n_new, 0xff, (size_t) (nd.C - n_new), h->stream));
     HIPCHK(h, hipMemsetAsync(nd.buffer + n_new, 0xff, (size_t) (nd.C - n_new), h->stream));
     HIPCHK(h, hipMemsetAsync(nd.acti + n_new, 0xff, (size_t) (nd.C - n_new), h->stream));
-    nd.err = &h->d_status->dev_error;
+    nd.err = &h->d_status_home->dev_error;
     k_gc_remap<<<blocks(n_new), 256, 0, h->stream>>>(nd, n_new, h->sz.NFACE, keep, o2n);
     HIPCHK(h, hipGetLastError());
     HIPCHK(h, hipStreamSynchronize(h->stream));
@@ -265,10 +265,11 @@ void batch_free(dacti_batch *b) {
     if (!b) return;
     if (b->stream) (void) hipStreamSynchronize(b->stream);
     batch_drop_graphs(b);
-    for (void *p : {(void *) b->d_items, (void *) b->d_mcs, (void *) b->d_toff, (void *) b->d_toffp, (void *) b->d_tilecnt, (void *) b->d_st,
-                    (void *) b->d_hist})
+    for (void *p : {(void *) b->d_items, (void *) b->d_mcs, (void *) b->d_toff, (void *) b->d_toffp, (void *) b->d_toffr,
+                    (void *) b->d_toff2, (void *) b->d_tilecnt, (void *) b->d_st, (void *) b->d_hist})
         if (p) (void) hipFree(p);
-    for (void *p : {(void *) b->h_items, (void *) b->h_toff, (void *) b->h_toffp, (void *) b->h_st, (void *) b->h_hist})
+    for (void *p : {(void *) b->h_items, (void *) b->h_toff, (void *) b->h_toffp, (void *) b->h_toffr, (void *) b->h_toff2,
+                    (void *) b->h_st, (void *) b->h_hist})
         if (p) (void) hipHostFree(p);
     for (auto &e : b->ev)
         if (e) (void) hipEventDestroy(e);
@@ -321,6 +322,10 @@ int batch_make(dacti_handle *const *hs, int n, hipStream_t stream, dacti_batch *
               hipMalloc(&b->d_mcs, sizeof(MConst) * (size_t) n) == hipSuccess &&
               hipMalloc(&b->d_toff, sizeof(int) * n1) == hipSuccess &&
               hipMalloc(&b->d_toffp, sizeof(int) * n1) == hipSuccess &&
+              hipMalloc(&b->d_toffr, sizeof(int) * n1) == hipSuccess &&
+              hipMalloc(&b->d_toff2, sizeof(int) * n1) == hipSuccess &&
+              hipHostMalloc(&b->h_toffr, sizeof(int) * n1) == hipSuccess &&
+              hipHostMalloc(&b->h_toff2, sizeof(int) * n1) == hipSuccess &&
               hipMalloc(&b->d_st, sizeof(Status) * (size_t) n) == hipSuccess &&
               hipMalloc(&b->d_hist, sizeof(unsigned long long) * 16 * (size_t) n) == hipSuccess &&
               hipHostMalloc(&b->h_items, sizeof(Item) * (size_t) n) == hipSuccess &&
@@ -343,6 +348,8 @@ int batch_make(dacti_handle *const *hs, int n, hipStream_t stream, dacti_batch *
     }
     std::memset(b->h_toff, 0, sizeof(int) * n1);
     std::memset(b->h_toffp, 0, sizeof(int) * n1);
+    std::memset(b->h_toffr, 0, sizeof(int) * n1);
+    std::memset(b->h_toff2, 0, sizeof(int) * n1);
     std::memset((void *) b->h_st, 0, sizeof(Status) * (size_t) n);
     *out = b;
     return DACTI_OK;
@@ -515,6 +522,7 @@ int dacti_create(const dacti_case *c, const dacti_tree *t, dacti_handle **out) {
         h->err = "small allocation failed";
         return fail(DACTI_E_MEMORY);
     }
+    h->d_status_home = h->d_status;
     const int64_t n = t->n_nodes;
     h->n = n;
     const int64_t C = n + std::max<int64_t>(n / 2, 65536);

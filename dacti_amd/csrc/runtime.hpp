@@ -81,6 +81,7 @@ struct dacti_handle {
     size_t tmp_bytes = 0;
     unsigned long long *d_hist = nullptr;
     Status *d_status = nullptr;
+    Status *d_status_home = nullptr;  // d_status as allocated (a batch swaps d_status while it steps)
     Status *h_status = nullptr;
     int32_t *h_nsel = nullptr;
     unsigned long long *h_hist = nullptr;
@@ -102,6 +103,8 @@ struct dacti_batch {
     MConst *d_mcs = nullptr;                     // constants of each instance
     int *h_toff = nullptr, *d_toff = nullptr;    // list tiles, n + 1 entries
     int *h_toffp = nullptr, *d_toffp = nullptr;  // pool tiles, n + 1 entries
+    int *h_toffr = nullptr, *d_toffr = nullptr;  // remesh: 256-entry list tiles of the meshing instances
+    int *h_toff2 = nullptr, *d_toff2 = nullptr;  // remesh: tiles of the rebuilt lists (upper bounds)
     int *d_tilecnt = nullptr;
     int64_t tilecnt_cap = 0;
     Status *h_st = nullptr, *d_st = nullptr;
@@ -179,6 +182,7 @@ inline Item base_item(dacti_batch *b, int i) {
     dacti_handle *h = b->hs[(size_t) i];
     Item it{};
     it.d = h->d;
+    it.d.err = &b->d_st[i].dev_error;
     it.list = h->d_list;
     it.rest = h->d_list2;
     it.st = b->d_st + i;

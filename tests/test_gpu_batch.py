@@ -13,7 +13,7 @@ import pytest
 from dacti_amd import scene, workloads
 from dacti_amd.integrator import Batch, Integrator
 from oracle.binding import Oracle
-from tests.helpers import compare_pools
+from tests.helpers import amr_sod_case, compare_pools
 
 pytestmark = pytest.mark.gpu
 
@@ -85,5 +85,28 @@ def test_batch_autotune_keeps_results():
         assert len(res) >= 4 and all(ms > 0 for *_, ms in res)
         best = min(res, key=lambda r: r[3])
         assert b.launch() == best[:3]
+    for o, g in pairs:
+        g.close()
+
+
+def test_batch_small_grid_spans_instances():
+    # a persistent grid much smaller than the tile count: one block walks
+    # tiles of several instances (pool histogram, selection, remesh phases)
+    cases = [workloads.periodic_box(level=lv, dim=2, seed=s) for lv, s in ((7, 8), (6, 9), (7, 10))]
+    cases.append(amr_sod_case(base=5, top=7))  # Euler 2D with split and unsplit
+    pairs = []
+    for k, case in enumerate(cases):
+        print(f"[batch] small grid: instance {k} tree", flush=True)
+        tree = scene.build_tree(case)
+        pairs.append((Oracle(case, tree), Integrator(case, tree)))
+    with Batch([g for _, g in pairs]) as b:
+        b.set_launch(block=64, grid=8, graphs=False)
+        for s in range(2):
+            _step(pairs, b)
+            _check_all(pairs, "small grid", s)
+        b.set_launch(block=256, grid=16, graphs=True)
+        for s in range(2, 4):
+            _step(pairs, b)
+            _check_all(pairs, "small grid (graphs)", s)
     for o, g in pairs:
         g.close()
