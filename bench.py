@@ -101,7 +101,8 @@ def batched(args):
 
     members = [Integrator(c, t) for (c, _), t in zip(fam, trees)]
     b = Batch(members)
-    b.autotune()
+    if not args.no_tune:
+        b.autotune()
     for w in range(warm):
         b.integrate()
         print(f"[bench] batched warmup/tuning step {w} done", file=sys.stderr, flush=True)
@@ -235,6 +236,9 @@ def main():
     ap.add_argument("--epochs", type=int, default=10)
     ap.add_argument("--instances", type=int, default=64)
     ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--no-tune", action="store_true", help="batched: keep the default launch (no autotuning)")
+    ap.add_argument("--plain-steps", type=int, default=None,
+                    help="batched: time the plain form over this many steps (default: --steps)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this engine runs on one GPU; --gpus {args.gpus} is not supported", file=sys.stderr)

@@ -1589,11 +1589,16 @@ struct OpFluxSimple {
         flux_simple_body<MODEL, D>(it.d, c, it.list, it.nl, li, it.L, it.dt, it.rest, &it.st->n_rest);
     }
 };
+// The set-aside cells are few and each is a long chain of dependent
+// loads: spread them round-robin over all of the item's tiles (blocks), so
+// that every CU takes some instead of the first few taking all.
 template <int MODEL, int D>
 struct OpFluxRest {
     __device__ __forceinline__ void operator()(const Item &it, const MConst &c, int li) const {
-        if (li >= it.st->n_rest) return;
-        flux_cell<MODEL, D>(it.d, c, it.rest[li], it.L, it.dt);
+        const int B = (int) blockDim.x, T = (it.nl + B - 1) / B;
+        const int r = (li % B) * T + li / B;
+        if (r >= it.st->n_rest) return;
+        flux_cell<MODEL, D>(it.d, c, it.rest[r], it.L, it.dt);
     }
 };
 template <int MODEL, int D>
