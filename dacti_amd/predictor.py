@@ -87,6 +87,36 @@ def labels(nodes_t, lv, nodes_h):
     return (np.sign(d) + 1).astype(np.int32)
 
 
+def labels_window(snaps, t, lv, h):
+    """What a remeshing cadence of h steps needs: 2 if the mesh is finer at
+    the leaf centre at any of the next h steps, 0 if it is coarser at all of
+    them, 1 otherwise."""
+    x = snaps[t]["center"][lv]
+    lev = snaps[t]["tree_level"][lv].astype(int)
+    finer = np.zeros(len(lv), bool)
+    coarser = np.ones(len(lv), bool)
+    for k in range(1, h + 1):
+        at = locate(snaps[t + k], x)
+        d = snaps[t + k]["tree_level"][at].astype(int) - lev
+        finer |= d > 0
+        coarser &= d < 0
+    return np.where(finer, 2, np.where(coarser, 0, 1)).astype(np.int32)
+
+
+def dilate(nodes, ids, rings=1):
+    """ids plus the active leaves reached through `rings` face-neighbour hops."""
+    F_MASK = F_LEAF | F_ACTIVE
+    cur = np.unique(np.asarray(ids, np.int64))
+    out = cur
+    for _ in range(rings):
+        nb = nodes["nbr"][cur].ravel()
+        nb = nb[nb > 0]
+        nb = nb[(nodes["flags"][nb] & F_MASK) == F_MASK]
+        cur = np.setdiff1d(np.unique(nb), out)
+        out = np.union1d(out, cur)
+    return out
+
+
 def indicator_guess(nodes, lv, case):
     """The method without the network: the estimator's present flags taken as
     the forecast (refine if the error is above max_error now, coarsen if
@@ -96,8 +126,10 @@ def indicator_guess(nodes, lv, case):
     return np.where(err > hi, 2, np.where(err < lo, 0, 1)).astype(np.int32)
 
 
-def run_samples(case, steps, horizon, tree=None, log=None):
-    """Step one case on the GPU; returns per-step (features, labels, indicator guess)."""
+def run_samples(case, steps, horizon, tree=None, log=None, window=False):
+    """Step one case on the GPU; returns per-step (features, labels, indicator
+    guess); window=True labels with labels_window (any refinement within the
+    horizon) instead of the mesh exactly `horizon` steps later."""
     g = Integrator(case, tree if tree is not None else scene.build_tree(case))
     snaps = [g.get_nodes()]
     for s in range(steps):
@@ -111,7 +143,8 @@ def run_samples(case, steps, horizon, tree=None, log=None):
     out = []
     for t in range(len(snaps) - horizon):
         lv, X = features(snaps[t], case)
-        out.append((X, labels(snaps[t], lv, snaps[t + horizon]), indicator_guess(snaps[t], lv, case)))
+        y = labels_window(snaps, t, lv, horizon) if window else labels(snaps[t], lv, snaps[t + horizon])
+        out.append((X, y, indicator_guess(snaps[t], lv, case)))
     return out
 
 
@@ -130,12 +163,13 @@ class Standardizer:
         return ((X - self.mu) / self.sd).astype(np.float32)
 
 
-def run_cadence(case, tree, steps, every=1, model=None, std=None, log=None):
+def run_cadence(case, tree, steps, every=1, model=None, std=None, log=None, rings=0):
     """Step one case with remeshing only in the global steps k with
     k % every == 0 (every = 1: the reference, which remeshes in every
     substep).  With a model, before each remeshing step the leaves it
-    forecasts as refined are flagged to split (the estimator still marks as
-    usual: it is the fallback).  Returns (final nodes, seconds, cell updates,
+    forecasts as refined (and, with rings > 0, their neighbours that many face
+    hops out) are flagged to split (the estimator still marks as usual: it is
+    the fallback).  Returns (final nodes, seconds, cell updates,
     flagged leaves)."""
     import time
     g = Integrator(case, tree)
@@ -148,6 +182,8 @@ def run_cadence(case, tree, steps, every=1, model=None, std=None, log=None):
             nodes = g.get_nodes()
             lv, X = features(nodes, case)
             ids = lv[model.predict(std(X)) == 2]
+            if rings:
+                ids = dilate(nodes, ids, rings)
             g.mark_split(ids)
             flagged += len(ids)
         g.set_remesh(on)
