@@ -86,3 +86,21 @@ def test_parity_multi3d_shock_drop():
 def test_parity_navier_stokes_3d_box():
     from tests.helpers import ns3_box
     _run_pair(ns3_box(level=3), 2, "navier-stokes 3d 8^3")
+
+
+def test_parity_gc_after_coarsening():
+    # an initially refined jump that every cell wants to coarsen: the active
+    # fraction falls below 0.2 in the first step and the pool is compacted
+    # (impl/acti_utils.hpp:58-74), after which node ids must still match
+    case = amr_sod_case(base=3, top=8, emin=1e10, emax=1e20)
+    tree = scene.build_tree(case)
+    n0 = tree.n
+    o = Oracle(case, tree)
+    g = Integrator(case, tree)
+    for s in range(3):
+        print(f"[parity] gc case: step {s}", flush=True)
+        o.integrate()
+        g.integrate()
+        compare_pools(g.get_nodes(), o.export())
+    assert g.n_nodes() < n0 // 4  # compacted
+    g.close()
