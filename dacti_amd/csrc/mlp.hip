@@ -40,20 +40,26 @@ inline int pad16(int x) { return (x + 15) / 16 * 16; }
 // (if given), ReLU (if relu), FP32 store to C[m*ldc + n], BF16 copy to Cb.
 // Lane layout of the 16x16x16 BF16 MFMA: lane l holds A(l%16, 4(l/16)+j) and
 // B(4(l/16)+j, l%16) for j < 4, and returns C(4(l/16)+i, l%16) for i < 4.
+// With ks > 1 the reduction is split into ks equal ranges (each a multiple
+// of 16): wave (tile, s) writes its partial tile to C + s * M * ldc, and
+// k_gemm_reduce adds the partials in order (no epilogue in that mode).
 __global__ void __launch_bounds__(256) k_gemm(const uint16_t *__restrict__ A, int64_t sam, int64_t sak,
                                               const uint16_t *__restrict__ B, int64_t sbk, int64_t sbn, int M, int N,
                                               int K, float *__restrict__ C, int64_t ldc, uint16_t *__restrict__ Cb,
-                                              const float *__restrict__ bias, int relu) {
+                                              const float *__restrict__ bias, int relu, int ks) {
     const int wave = (int) ((blockIdx.x * (int64_t) blockDim.x + threadIdx.x) >> 6);
     const int lane = threadIdx.x & 63;
     const int tn = N / 16, tiles = (M / 16) * tn;
-    if (wave >= tiles) return;
-    const int m0 = (wave / tn) * 16, n0 = (wave % tn) * 16;
+    if (wave >= tiles * ks) return;
+    const int tile = wave % tiles, part = wave / tiles;
+    const int m0 = (tile / tn) * 16, n0 = (tile % tn) * 16;
     const int r = lane & 15, kq = (lane >> 4) * 4;
+    const int kl = K / ks, kb = part * kl;
     f4 c = {0.0f, 0.0f, 0.0f, 0.0f};
-    const uint16_t *pa = A + (int64_t) (m0 + r) * sam + (int64_t) kq * sak;
-    const uint16_t *pb = B + (int64_t) kq * sbk + (int64_t) (n0 + r) * sbn;
-    for (int k0 = 0; k0 < K; k0 += 16) {
+    const uint16_t *pa = A + (int64_t) (m0 + r) * sam + (int64_t) (kb + kq) * sak;
+    const uint16_t *pb = B + (int64_t) (kb + kq) * sbk + (int64_t) (n0 + r) * sbn;
+    C += (int64_t) part * M * ldc;
+    for (int k0 = 0; k0 < kl; k0 += 16) {
         s4 a, b;
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
@@ -71,6 +77,15 @@ __global__ void __launch_bounds__(256) k_gemm(const uint16_t *__restrict__ A, in
         C[(int64_t) m * ldc + n] = v;
         if (Cb) Cb[(int64_t) m * ldc + n] = to_bf16(v);
     }
+}
+
+// C[i] = sum over s of part[s * n + i], in s order
+__global__ void k_gemm_reduce(const float *__restrict__ part, int ks, int64_t n, float *__restrict__ C) {
+    const int64_t i = blockIdx.x * (int64_t) blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float v = part[i];
+    for (int s = 1; s < ks; ++s) v += part[(int64_t) s * n + i];
+    C[i] = v;
 }
 
 __global__ void k_to_bf16(const float *__restrict__ x, uint16_t *__restrict__ y, int64_t n) {
@@ -174,6 +189,8 @@ struct dacti_mlp {
     std::vector<float *> Z;
     std::vector<uint16_t *> A16;
     float *dZ = nullptr, *dA = nullptr, *lossb = nullptr;
+    float *part = nullptr;  // split-reduction partial tiles
+    int64_t part_cap = 0;
     uint16_t *dZ16 = nullptr;
     int32_t *yb = nullptr;
     hipStream_t s = nullptr;
@@ -230,10 +247,25 @@ int ensure_rows(dacti_mlp *m, int rows) {
     return 0;
 }
 
+// A long reduction over few output tiles (the weight gradients: K = batch
+// rows) is split so that enough wavefronts run; otherwise one wave per tile.
 int gemm(dacti_mlp *m, const uint16_t *A, int64_t sam, int64_t sak, const uint16_t *B, int64_t sbk, int64_t sbn, int M,
          int N, int K, float *C, int64_t ldc, uint16_t *Cb, const float *bias, int relu) {
-    const int64_t waves = (int64_t) (M / 16) * (N / 16);
-    k_gemm<<<nblk(waves * 64), 256, 0, m->s>>>(A, sam, sak, B, sbk, sbn, M, N, K, C, ldc, Cb, bias, relu);
+    const int64_t tiles = (int64_t) (M / 16) * (N / 16);
+    int ks = 1;
+    if (!bias && !relu && !Cb)
+        while (ks < 32 && tiles * ks < 2048 && K / (2 * ks) >= 64 && (K / 16) % (2 * ks) == 0) ks *= 2;
+    const int64_t need = (int64_t) ks * M * ldc;
+    if (ks > 1 && need > m->part_cap) {
+        MCHK(m, hipStreamSynchronize(m->s));
+        if (m->part) (void) hipFree(m->part);
+        m->part = nullptr;
+        MCHK(m, hipMalloc((void **) &m->part, sizeof(float) * (size_t) need));
+        m->part_cap = need;
+    }
+    k_gemm<<<nblk(tiles * ks * 64), 256, 0, m->s>>>(A, sam, sak, B, sbk, sbn, M, N, K, ks > 1 ? m->part : C, ldc, Cb,
+                                                   bias, relu, ks);
+    if (ks > 1) k_gemm_reduce<<<nblk((int64_t) M * ldc), 256, 0, m->s>>>(m->part, ks, (int64_t) M * ldc, C);
     m->flops += 2.0 * M * N * K;
     MCHK(m, hipGetLastError());
     return 0;
@@ -334,7 +366,7 @@ void destroy(dacti_mlp *m) {
         if (p) (void) hipFree(p);
     for (auto p : m->A16)
         if (p) (void) hipFree(p);
-    for (void *p : {(void *) m->X16, (void *) m->dZ, (void *) m->dA, (void *) m->dZ16, (void *) m->yb,
+    for (void *p : {(void *) m->X16, (void *) m->dZ, (void *) m->dA, (void *) m->dZ16, (void *) m->yb, (void *) m->part,
                     (void *) m->lossb})
         if (p) (void) hipFree(p);
     if (m->e0) (void) hipEventDestroy(m->e0);
