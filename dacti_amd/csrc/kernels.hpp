@@ -1397,11 +1397,6 @@ __device__ void interpolated_cell_acc(const Dev &d, int32_t idx, double &t, doub
     for (int e = 0; e < S::NGX; ++e) ts[e] = ts[e] + cts[e];
 }
 
-static __global__ void k_clear_marks(Dev d, const int32_t *__restrict__ list, int nl) {
-    const int li = blockIdx.x * blockDim.x + threadIdx.x;
-    if (li >= nl) return;
-    d.mark[list[li]] = 0;
-}
 
 // update_neighbors (domain.hpp:249-293) for every active leaf
 template <int D>

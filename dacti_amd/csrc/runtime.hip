@@ -1,11 +1,11 @@
 // runtime.hip -- host runtime and C ABI of the MI355X leaf-update engine.
 //
-// The host side restates Integrator (src/lib/integrator/integrator.hpp) and
-// its integrate() schedule (impl/integrate.hpp:11-87); every per-cell phase
-// runs as a HIP kernel over the level's cluster list (kernels.hpp).  The
-// host keeps only scalars: the LTS counters and times, the per-level
-// cluster sizes and the node count.  The model-specific code is compiled
-// once per model in variant_*.hip.
+// Pool allocation, construction, garbage collection, batches and the C ABI.
+// The integrate() schedule itself (impl/integrate.hpp:11-87) is in
+// batch.hpp: a single integrator is stepped as a batch of one.  The host
+// keeps only scalars per instance: the LTS counters and times, the per-level
+// cluster sizes and the node count.  The model-specific code is compiled once
+// per model in variant_*.hip.
 #include "runtime.hpp"
 
 namespace dacti_rt {
@@ -111,24 +111,6 @@ int ensure_capacity(dacti_handle *h, int64_t need) {
 }
 
 
-
-// sorted list of the node ids whose cluster is L (acti_clusters[L])
-int build_list(dacti_handle *h, int L, int64_t *nl, int32_t *out) {
-    hipcub::CountingInputIterator<int32_t> it(1);
-    size_t bytes = h->tmp_bytes;
-    HIPCHK(h, hipcub::DeviceSelect::If(h->d_tmp, bytes, it, out ? out : h->d_list, h->d_nsel, (int) (h->n - 1),
-                                       ClusterIs{h->d.cluster, L}, h->stream));
-    HIPCHK(h, hipMemcpyAsync(h->h_nsel, h->d_nsel, sizeof(int32_t), hipMemcpyDeviceToHost, h->stream));
-    HIPCHK(h, hipStreamSynchronize(h->stream));
-    *nl = *h->h_nsel;
-    return DACTI_OK;
-}
-
-int read_status(dacti_handle *h) {
-    HIPCHK(h, hipMemcpyAsync(h->h_status, h->d_status, sizeof(Status), hipMemcpyDeviceToHost, h->stream));
-    HIPCHK(h, hipStreamSynchronize(h->stream));
-    return DACTI_OK;
-}
 
 int refresh_counts(dacti_handle *h) {
     HIPCHK(h, hipMemsetAsync(h->d_hist, 0, sizeof(unsigned long long) * 16, h->stream));

@@ -1,11 +1,11 @@
 // runtime.hpp -- host runtime of the MI355X leaf-update engine (shared by
 // the per-model translation units and the C ABI).
 //
-// The host side restates Integrator (src/lib/integrator/integrator.hpp) and
-// its integrate() schedule (impl/integrate.hpp:11-87); every per-cell phase
-// runs as a HIP kernel over the level's cluster list (kernels.hpp).  The
-// host keeps only scalars: the LTS counters and times, the per-level
-// cluster sizes and the node count.
+// The host side restates Integrator (src/lib/integrator/integrator.hpp):
+// the handle (one integrator's state), the batch (instances stepped
+// together, batch.hpp) and the per-model construction (Ops::init).  Every
+// per-cell phase runs as a HIP kernel over the level's cluster list
+// (kernels.hpp); the host keeps only scalars per instance.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <hipcub/hipcub.hpp>
@@ -145,16 +145,8 @@ namespace dacti_rt {
 
 inline unsigned blocks(int64_t n, int b = 256) { return (unsigned) ((n + b - 1) / b); }
 
-struct ClusterIs {
-    const int8_t *cluster;
-    int level;
-    __host__ __device__ bool operator()(const int32_t &i) const { return cluster[i] == level; }
-};
-
 int alloc_pool(dacti_handle *h, int64_t C, Dev *old, int64_t n);
 int ensure_capacity(dacti_handle *h, int64_t need);
-int build_list(dacti_handle *h, int L, int64_t *nl, int32_t *out = nullptr);
-int read_status(dacti_handle *h);
 int refresh_counts(dacti_handle *h);
 int gc(dacti_handle *h);
 int batch_ensure_tiles(dacti_batch *b, int64_t tiles);
