@@ -223,6 +223,62 @@ def learned(args):
     print(json.dumps(out), flush=True)
 
 
+def bubble(args):
+    """2D shock-bubble (five-equation model): 64x32 base quadtree + `--extra`
+    levels, Mach 1.22 shock in air hitting a helium-like bubble (gamma 1.67,
+    density 0.138, radius 0.25), HLLC + MUSCL minmod, CFL 0.5, LTS on,
+    remeshing in every substep: the plain method, GPU vs the CPU port."""
+    from dacti_amd import scene, workloads
+    from dacti_amd.integrator import Integrator
+    case = workloads.shock_bubble(base=6, extra=args.extra)
+    tree = scene.build_tree(case)
+    print(f"[bench] shock-bubble tree: {tree.n} nodes, {len(tree.leaves())} leaves", file=sys.stderr, flush=True)
+    g = Integrator(case, tree)
+    if args.graphs:
+        g.set_launch(block=128, grid=2048, graphs=True)
+    for w in range(args.warmup):
+        g.integrate()
+    s0 = g.stats()
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        if g.integrate()[1]:
+            raise RuntimeError("NaN state (force_break)")
+        if k % 10 == 9:
+            print(f"[bench] step {k} done", file=sys.stderr, flush=True)
+    el = time.perf_counter() - t0
+    s1 = g.stats()
+    updates = s1.cell_updates - s0.cell_updates
+    base = None
+    if not args.no_cpu_baseline:
+        from oracle.binding import Oracle
+        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        o = Oracle(case, tree)
+        t1 = time.perf_counter()
+        n_cpu = 0
+        while time.perf_counter() - t1 < 10.0 and n_cpu < args.warmup + args.steps:
+            o.integrate()
+            n_cpu += 1
+        el_cpu = time.perf_counter() - t1
+        base = {"value": o.cell_updates / el_cpu, "unit": "cell-updates/s", "cores": threads, "kind": "port",
+                "sample": f"the first {n_cpu} global steps from the same initial state ({o.cell_updates} updates in "
+                          f"{el_cpu:.1f} s, OpenMP {threads} threads)"}
+        o.close()
+    out = {
+        "metric": "cell-updates/s (HLLC+MUSCL FP64) & %HBM roofline; learned/batched vs plain",
+        "value": updates / el, "unit": "cell-updates/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": el * 1e3 / args.steps, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"2D shock-bubble (five-equation model), 64x32 base + {args.extra} levels, "
+                               "remesh every substep, LTS on",
+                   "nodes": s1.n_nodes, "active_leaves": s1.n_active, "max_acti_level": s1.max_acti_level,
+                   "cell_updates_timed": updates, "graphs": bool(args.graphs)},
+        "roofline": None,
+        "cpu_baseline": base,
+    }
+    g.close()
+    print(json.dumps(out), flush=True)
+
+
 def drop(args):
     """3D shock-drop (five-equation stiffened gas, SI units): Mach 2.4 shock
     in air hitting a 1 mm water drop in a 16 x 8 x 8 mm box, octree levels
@@ -289,9 +345,11 @@ def main():
     ap.add_argument("--level", type=int, default=8)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-probe", action="store_true", help="run a 1 GiB device copy first (counter calibration)")
-    ap.add_argument("--workload", choices=("box", "batched", "learned", "drop"), default="box",
+    ap.add_argument("--workload", choices=("box", "batched", "learned", "drop", "bubble"), default="box",
                     help="box: 3D periodic Euler box (default); batched: shock-bubble instances, batch vs plain; "
-                         "learned: refinement predictor vs the estimator; drop: 3D shock-drop with AMR")
+                         "learned: refinement predictor vs the estimator; drop: 3D shock-drop with AMR; "
+                         "bubble: 2D shock-bubble with AMR")
+    ap.add_argument("--graphs", action="store_true", help="bubble: persistent grid and graph replay")
     ap.add_argument("--base", type=int, default=5)
     ap.add_argument("--extra", type=int, default=4)
     ap.add_argument("--horizon", type=int, default=16)
@@ -316,6 +374,9 @@ def main():
         return
     if args.workload == "drop":
         drop(args)
+        return
+    if args.workload == "bubble":
+        bubble(args)
         return
 
     from dacti_amd import scene, workloads

@@ -759,6 +759,13 @@ const char *dacti_batch_last_error(dacti_batch *b) {
     return b->err.c_str();
 }
 
+int dacti_set_launch(dacti_handle *h, int32_t block, int32_t grid, int32_t graphs) {
+    if (!h) return DACTI_E_ARG;
+    const int rc = dacti_batch_set_launch(h->solo, block, grid, graphs);
+    if (rc) h->err = h->solo->err;
+    return rc;
+}
+
 int dacti_set_remesh(dacti_handle *h, int32_t on) {
     if (!h) return DACTI_E_ARG;
     h->remesh = on != 0;

@@ -91,6 +91,10 @@ class Integrator:
         self._check(self.lib.dacti_get_stats(self.h, C.byref(s)), "get_stats")
         return s
 
+    def set_launch(self, block=256, grid=0, graphs=False):
+        """Launch configuration of this integrator's stepping (results do not depend on it)."""
+        self._check(self.lib.dacti_set_launch(self.h, block, grid, int(bool(graphs))), "set_launch")
+
     def set_remesh(self, on):
         """Pause (False) or resume (True) refinement marks; an extension of
         the reference, which remeshes in every substep."""

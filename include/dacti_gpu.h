@@ -176,6 +176,12 @@ int dacti_get_stats(dacti_handle *h, dacti_stats *s);
 /* turn HIP-event timing of the per-substep kernels on or off */
 int dacti_set_timing(dacti_handle *h, int32_t on);
 
+/* Launch configuration of this handle's own stepping (see
+ * dacti_batch_set_launch): block 64/128/256, persistent grid (multiple of 8,
+ * 0 = one block per tile), graphs (needs grid > 0).  Results do not depend
+ * on it; small adaptive cases gain from graph replay. */
+int dacti_set_launch(dacti_handle *h, int32_t block, int32_t grid, int32_t graphs);
+
 /* Remeshing cadence (an extension; the reference remeshes in every substep,
  * impl/integrate.hpp:59): with on = 0 the sweep sets no split or unsplit
  * marks, so the mesh stays as it is while the LTS levels still adapt. */
