@@ -62,6 +62,15 @@ __device__ __forceinline__ bool is_leaf(const Dev &d, int32_t i) { return d.flag
 // L2).  Give XCD x a contiguous run of logical blocks, so a run of
 // consecutive list entries (spatial neighbours in Morton order) shares one
 // L2.  A bijection of [0, nb).
+// a[k] for a runtime k < N, as a chain of selects (no indexed private array)
+template <int N>
+__device__ __forceinline__ int pick_i(const int *a, int k) {
+    int v = a[0];
+#pragma unroll
+    for (int q = 1; q < N; ++q) v = k == q ? a[q] : v;
+    return v;
+}
+
 __device__ __forceinline__ int xcd_block(int b, int nb) {
     const int q = nb >> 3, r = nb & 7;
     const int x = b & 7, k = b >> 3;
@@ -603,10 +612,18 @@ __device__ __forceinline__ void flux_simple_body(const Dev &d, const MConst &c, 
     const int64_t C = d.C;
     const int32_t y = list[li];
     const int ylev = d.level[y];
-    // classify: every face a boundary or a same-level leaf?
+    // classify: every face a boundary or a same-level leaf?  The neighbour
+    // ids and their LTS level and cluster are gathered here for all faces at
+    // once, so the face loop below waits only for the neighbour states.
     bool simple = true;
+    int32_t nbs[2 * D];
+    int nac[2 * D], ncls[2 * D];
+#pragma unroll
     for (int f = 0; f < 2 * D; ++f) {
         const int32_t nb = d.nbr[f * C + y];
+        nbs[f] = nb;
+        nac[f] = nb ? d.acti[nb] : 0;
+        ncls[f] = nb ? d.cluster[nb] : -1;
         if (nb != 0 && (!(d.flags[nb] & F_LEAF) || d.level[nb] != ylev)) simple = false;
     }
     if (!simple) {
@@ -641,7 +658,7 @@ __device__ __forceinline__ void flux_simple_body(const Dev &d, const MConst &c, 
 #pragma unroll 1
         for (int dir = 0; dir < 2; ++dir) {
             const int f = 2 * dim + dir;
-            const int32_t nb = d.nbr[f * C + y];
+            const int32_t nb = pick_i<2 * D>(nbs, f);
             double cc[NF], F[NF];
 #pragma unroll
             for (int e = 0; e < NF; ++e) cc[e] = 0.0;
@@ -650,8 +667,8 @@ __device__ __forceinline__ void flux_simple_body(const Dev &d, const MConst &c, 
 #pragma unroll
                 for (int e = 0; e < NF; ++e) cc[e] = cc[e] - F[e];
             } else {
-                const int nacti = d.acti[nb];
-                const int ncl = d.cluster[nb];
+                const int nacti = pick_i<2 * D>(nac, f);
+                const int ncl = pick_i<2 * D>(ncls, f);
                 if (nacti > L) {
 #pragma unroll
                     for (int e = 0; e < NF; ++e) {
