@@ -348,12 +348,8 @@ struct BOps {
             for (int i = 0; i < n; ++i)
                 if (b->h_items[i].nl > 0 && b->h_st[i].n_marked > 0) marked.push_back(i);
             if (!marked.empty()) {
+                // uploads the grown pools' items and pool tiles itself
                 rc = remesh_all(b, L, marked, changed);
-                if (rc) return rc;
-                // pools may have grown or moved: new items and pool tiles
-                rc = sync(b);
-                if (rc) return rc;
-                rc = upload_tables(b, changed);
                 if (rc) return rc;
             }
             any_list = false;

@@ -209,7 +209,8 @@ const char *dacti_last_error(dacti_handle *h);
  * substep, so each evolves as it would alone (same results, bit for bit).
  * While a batch exists its handles must be stepped only through it; their
  * read-side calls (export, stats, n_active) stay valid.  Destroying the
- * batch leaves the handles intact.
+ * batch leaves the handles intact; destroy a batch before any of its
+ * handles.
  */
 typedef struct dacti_batch dacti_batch;
 
