@@ -46,6 +46,8 @@ struct BOps {
             k_batch<<<gl, B, 0, st>>>(it, b->d_mcs, b->d_toff, n, OpSmoothCurvature<MODEL, D>{});
             k_batch<<<gl, B, 0, st>>>(it, b->d_mcs, b->d_toff, n, OpStoreK{S::IKAPPA});
         }
+        // split readiness and unsplit detection of the marked cells
+        k_batch<<<gl, B, 0, st>>>(it, b->d_mcs, b->d_toff, n, OpMark2<D>{});
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) {
             b->err = std::string("head launch: ") + hipGetErrorString(e);
@@ -191,16 +193,9 @@ struct BOps {
         }
         b->h_toffr[n] = (int) tr;
         b->h_toff2[n] = 0;
-        int rc = upload_tables(b, changed);
-        if (rc) return rc;
-        k_batch<<<list_grid(b), B, 0, st>>>(b->d_items, b->d_mcs, b->d_toff, n, OpMark2<D>{});
-        if (hipGetLastError() != hipSuccess) {
-            b->err = "remesh launch failed";
-            return DACTI_E_HIP;
-        }
-        rc = copy_status(b);
-        if (rc) return rc;
-        // pool growth on the host (rare reallocation), pool size before the splits
+        int rc = 0;
+        // split counts from the head (OpMark2); pool growth on the host (rare
+        // reallocation), pool size before the splits
         int64_t t2 = 0;
         for (int i = 0; i < n; ++i) {
             b->h_toff2[i] = (int) t2;
@@ -335,6 +330,8 @@ struct BOps {
             }
             if (deferred) {
                 k_batch<<<list_grid(b), b->block, 0, b->stream>>>(b->d_items, b->d_mcs, b->d_toff, n, OpSweep<MODEL, D>{1});
+                k_clear_split_counts<<<(unsigned) ((n + 63) / 64), 64, 0, b->stream>>>(b->d_items, n);
+                k_batch<<<list_grid(b), b->block, 0, b->stream>>>(b->d_items, b->d_mcs, b->d_toff, n, OpMark2<D>{});
                 if (hipGetLastError() != hipSuccess ||
                     hipMemcpyAsync(b->h_st, b->d_st, sizeof(Status) * (size_t) n, hipMemcpyDeviceToHost, b->stream) !=
                         hipSuccess) {

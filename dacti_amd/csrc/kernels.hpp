@@ -1882,13 +1882,23 @@ static __global__ void __launch_bounds__(256) k_pool_hist(const Item *__restrict
 // ------------------------------------------------- batched remesh phases
 // (impl/adaptive_refinement.hpp:97-322) for the instances with marks
 // (Item::meshing); the others return at once.
+// Launched with the substep's head for every instance that has marks (the
+// sweep counted them), so the remesh needs no wait of its own; run again,
+// after clearing its counts, when deferred cells received marks later.
 template <int D>
 struct OpMark2 {
     __device__ __forceinline__ void operator()(const Item &it, const MConst &, int li) const {
-        if (!it.meshing) return;
+        if (it.st->n_marked == 0 || it.st->nan_flag) return;
         mark2_body<D>(it.d, it.list, it.nl, li, it.L, it.rn, it.st);
     }
 };
+
+static __global__ void k_clear_split_counts(const Item *__restrict__ items, int n_items) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n_items) return;
+    items[i].st->n_ready_split = 0;
+    items[i].st->n_new_blocks = 0;
+}
 
 // exclusive scan of the new-block flags over each item's list: per list
 // tile a count (tiles of 256 entries), one scan per item, then the offsets
