@@ -33,6 +33,7 @@ struct Dev {
     double *slope;                          // [(d*NE + e)*C + i]
     double *grad;                           // [(d*NGX + e)*C + i]
     double *acc;                            // [(f*NF + e)*C + i]
+    double *dep;                            // [(d*NF + e)*C + i] flux of a fine cell's face toward a coarser receiver
     double *error, *ctime, *max_dt, *cfl;
     double *scratch;                        // C doubles (smoothing / reductions)
     int32_t *err;                           // device error word (subtree deeper than supported)
@@ -538,25 +539,11 @@ __device__ void flux_cell(const Dev &d, const MConst &c, int32_t y, int L, doubl
                                 a = dir == 1 ? a + F[e] : a - F[e];
                             }
                         } else {
-                            // coarser receiver: the first computing sibling deposits
-                            // all computing siblings' fluxes in sub-face order
-                            const int32_t rn = d.nbr[fo * C + nb];
-                            int32_t first = 0;
-                            for_sub_face_leaves<D>(d, rn, dim, dir, [&](int32_t x) {
-                                if (first == 0 && d.cluster[x] == L && d.nbr[f * C + x] == nb) first = x;
-                            });
-                            if (first == y) {
-                                for_sub_face_leaves<D>(d, rn, dim, dir, [&](int32_t x) {
-                                    if (d.cluster[x] == L && d.nbr[f * C + x] == nb) {
-                                        double G[NF];
-                                        owner_face_flux_with<MODEL, D>(d, c, x, sn, nb, dim, dir, dt, G);
-                                        for (int e = 0; e < NF; ++e) {
-                                            double &a = d.acc[(fo * NF + e) * C + nb];
-                                            a = dir == 1 ? a + G[e] : a - G[e];
-                                        }
-                                    }
-                                });
-                            }
+                            // coarser receiver: kept here and deposited by
+                            // deposit_coarse (the first computing sibling adds
+                            // all siblings' fluxes in sub-face order)
+#pragma unroll
+                            for (int e = 0; e < NF; ++e) d.dep[(dim * NF + e) * C + y] = F[e];
                         }
                     }
                 }
@@ -596,6 +583,42 @@ __global__ void __launch_bounds__(256) k_flux(Dev d, MConst c, const int32_t *__
     const int nl = *n;
     for (int li = blockIdx.x * blockDim.x + threadIdx.x; li < nl; li += gridDim.x * blockDim.x)
         flux_cell<MODEL, D>(d, c, list[li], L, dt);
+}
+
+// The deposits of a set-aside cell toward coarser receivers outside the
+// cluster (impl/flux.hpp:82-91 with the oracle's order): for each such face
+// the first computing sibling of the sub-face group adds the siblings'
+// fluxes (stored by flux_cell) to the receiver's face accumulator in
+// sub-face order.  Runs after every flux of the substep is computed.
+template <int MODEL, int D>
+__device__ void deposit_coarse(const Dev &d, int32_t y, int L) {
+    using S = Sz<MODEL, D>;
+    constexpr int NF = S::NF;
+    const int64_t C = d.C;
+    const int ylev = d.level[y];
+#pragma unroll 1
+    for (int f = 0; f < 2 * D; ++f) {
+        const int dim = f >> 1, dir = f & 1;
+        const int32_t nb = d.nbr[f * C + y];
+        if (nb == 0 || !(d.flags[nb] & F_LEAF)) continue;
+        if (d.level[nb] >= ylev || d.acti[nb] > L || d.cluster[nb] == L) continue;
+        const int fo = 2 * dim + (1 - dir);
+        const int32_t rn = d.nbr[fo * C + nb];
+        int32_t first = 0;
+        for_sub_face_leaves<D>(d, rn, dim, dir, [&](int32_t x) {
+            if (first == 0 && d.cluster[x] == L && d.nbr[f * C + x] == nb) first = x;
+        });
+        if (first != y) continue;
+        for_sub_face_leaves<D>(d, rn, dim, dir, [&](int32_t x) {
+            if (d.cluster[x] == L && d.nbr[f * C + x] == nb) {
+                for (int e = 0; e < NF; ++e) {
+                    double &a = d.acc[(fo * NF + e) * C + nb];
+                    const double g = d.dep[(dim * NF + e) * C + x];
+                    a = dir == 1 ? a + g : a - g;
+                }
+            }
+        });
+    }
 }
 
 // Lean flux kernel for cells whose faces are all domain boundaries or
@@ -1631,6 +1654,16 @@ struct OpFluxRest {
         flux_cell<MODEL, D>(it.d, c, it.rest[r], it.L, it.dt);
     }
 };
+template <int MODEL, int D>
+struct OpDepositCoarse {
+    __device__ __forceinline__ void operator()(const Item &it, const MConst &, int li) const {
+        const int B = (int) blockDim.x, T = (it.nl + B - 1) / B;
+        const int r = (li % B) * T + li / B;
+        if (r >= it.st->n_rest) return;
+        deposit_coarse<MODEL, D>(it.d, it.rest[r], it.L);
+    }
+};
+
 template <int MODEL, int D>
 struct OpUpdate {
     __device__ __forceinline__ void operator()(const Item &it, const MConst &c, int li) const {

@@ -27,6 +27,7 @@ std::vector<Field> fields_of(dacti_handle *h, Dev &d) {
         {(void **) &d.tot, 8, s.NF},         {(void **) &d.k, 8, s.NPX},
         {(void **) &d.t_slope, 8, s.NGX},    {(void **) &d.slope, 8, s.D * s.NE},
         {(void **) &d.grad, 8, s.D * s.NGX}, {(void **) &d.acc, 8, s.NFACE * s.NF},
+        {(void **) &d.dep, 8, s.D * s.NF},
         {(void **) &d.error, 8, 1},          {(void **) &d.ctime, 8, 1},
         {(void **) &d.max_dt, 8, 1},         {(void **) &d.cfl, 8, 1},
         {(void **) &d.scratch, 8, 1},
