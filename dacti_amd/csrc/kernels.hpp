@@ -576,15 +576,6 @@ __device__ void flux_cell(const Dev &d, const MConst &c, int32_t y, int L, doubl
     for (int e = 0; e < NF; ++e) d.tot[e * C + y] = tot[e];
 }
 
-// general flux kernel over the cells the lean kernel set aside (count in *n)
-template <int MODEL, int D>
-__global__ void __launch_bounds__(256) k_flux(Dev d, MConst c, const int32_t *__restrict__ list,
-                                               const int32_t *__restrict__ n, int L, double dt) {
-    const int nl = *n;
-    for (int li = blockIdx.x * blockDim.x + threadIdx.x; li < nl; li += gridDim.x * blockDim.x)
-        flux_cell<MODEL, D>(d, c, list[li], L, dt);
-}
-
 // The deposits of a set-aside cell toward coarser receivers outside the
 // cluster (impl/flux.hpp:82-91 with the oracle's order): for each such face
 // the first computing sibling of the sub-face group adds the siblings'
@@ -625,7 +616,7 @@ __device__ void deposit_coarse(const Dev &d, int32_t y, int L) {
 // same-tree-level leaves (every cell of a uniform tree, most cells of an
 // adaptive one): one flux evaluation site per face, no sub-face walks.
 // A cell with any other face is appended to `rest` (order irrelevant:
-// cells are independent) and handled by k_flux.
+// cells are independent) and handled by OpFluxRest.
 template <int MODEL, int D>
 __device__ __forceinline__ void flux_simple_body(const Dev &d, const MConst &c, const int32_t *__restrict__ list, int nl, int li,
                                                       int L, double dt, int32_t *rest, int32_t *n_rest) {
