@@ -1447,7 +1447,53 @@ __device__ void interpolated_cell_acc(const Dev &d, int32_t idx, double &t, doub
 }
 
 
-// update_neighbors (domain.hpp:249-293) for every active leaf
+// dom_find started at a node h: the root descent passes through every live
+// (non-garbage) node that contains the point, and stops at the first leaf or
+// node of level max_level, so starting at such an h (after climbing out of
+// garbage blocks and above max_level) gives the same node.  A point a cell
+// centre lies at least half a cell away from every coarser node's faces and
+// centre, so comparing it with h's centre picks the same children as the
+// scaled coordinates of dom_find.  Falls back to dom_find when h does not
+// contain the point.
+template <int D>
+__device__ inline int32_t dom_find_from(const Dev &d, const MConst &c, double *p, int max_level, int32_t h, int64_t n) {
+    for (int i = 0; i < D; ++i) {
+        const double hw = 0.5 * c.active_size[i];
+        if (p[i] < -hw || p[i] > hw) {
+            if (c.periodic[i]) {
+                if (p[i] < -hw) p[i] += c.active_size[i];
+                else p[i] -= c.active_size[i];
+            } else {
+                return 0;
+            }
+        }
+    }
+    if (h >= 1 && h < n) {
+        const int64_t C = d.C;
+        for (int g = 0; g < 64 && h > 1 && ((d.flags[h] & F_GARBAGE) || d.level[h] > max_level); ++g) h = d.parent[h];
+        bool in = h >= 1 && !(d.flags[h] & F_GARBAGE) && d.level[h] <= max_level;
+        if (in) {
+            const double hs = 0.5 * d.size[h];
+            for (int i = 0; i < D; ++i) in = in && fabs(p[i] - d.center[i * C + h]) < hs;
+        }
+        if (in) {
+            for (int guard = 0; guard < 64; ++guard) {
+                if ((d.flags[h] & F_LEAF) || d.level[h] == max_level) return h;
+                int q = 0;
+                for (int i = 0; i < D; ++i)
+                    if (p[i] > d.center[i * C + h]) q |= 1 << i;
+                h = d.child_base[h] + q;
+            }
+            return h;
+        }
+    }
+    return dom_find<D>(d, c, p, max_level);
+}
+
+// update_neighbors (domain.hpp:249-293) for every active leaf.  The search
+// for each face's neighbour starts at the face's previous neighbour (or, for
+// a new child, at its parent for an inner face and at the parent's neighbour
+// for an outer one): after a remesh it is the answer or one level off.
 template <int D>
 __device__ __forceinline__ void neighbors_node(const Dev &d, const MConst &c, int64_t i, int64_t n) {
     if (i < 1 || i >= n) return;
@@ -1456,14 +1502,20 @@ __device__ __forceinline__ void neighbors_node(const Dev &d, const MConst &c, in
     const int64_t C = d.C;
     const int lev = d.level[i];
     const double s = d.size[i];
+    const int32_t par = d.parent[i];
     for (int dim = 0; dim < D; ++dim)
         for (int dir = 0; dir < 2; ++dir) {
             const int f = 2 * dim + dir;
             if (d.bc[f * C + i]) continue;
             double loc[D];
             for (int a = 0; a < D; ++a) loc[a] = d.center[a * C + i];
+            int32_t hint = d.nbr[f * C + i];
+            if (hint == 0 && par >= 1 && par < n) {
+                const bool inner = (dir == 1) == (loc[dim] < d.center[dim * C + par]);
+                hint = inner ? par : d.nbr[f * C + par];
+            }
             loc[dim] = loc[dim] + s * (2 * dir - 1.0);
-            const int32_t nb = dom_find<D>(d, c, loc, lev);
+            const int32_t nb = dom_find_from<D>(d, c, loc, lev, hint, n);
             if (!nb || !(d.flags[nb] & F_ACTIVE)) continue;
             d.nbr[f * C + i] = nb;
             if (d.level[nb] < lev - 1) atomic_or_u8(&d.flags[nb], F_SHOULD_SPLIT);

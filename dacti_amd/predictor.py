@@ -62,7 +62,7 @@ def features(nodes, case):
     for off in itertools.product((-1, 0, 1), repeat=dim):
         if not any(off):
             continue
-        pts = ctr[lv] + 2 * size[lv, None] * np.asarray(off, float)  # size is the half width
+        pts = ctr[lv] + size[lv, None] * np.asarray(off, float)  # size is the full width
         inside = np.ones(len(lv), bool)
         for d in range(dim):
             if case.periodic[d]:
