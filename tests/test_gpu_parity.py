@@ -83,6 +83,12 @@ def test_parity_multi3d_shock_drop():
     _run_pair(case, 2, "multi3d shock-drop 8x4x4+2")
 
 
+def test_parity_multi3d_shock_drop_levels_4_7():
+    # the shock-drop workload itself (Mach 2.4, 1 mm drop) at octree levels
+    # 4-7: ~27 M cell updates over three global steps, pools bitwise equal
+    _run_pair(workloads.shock_drop(base=4, extra=3), 3, "multi3d shock-drop levels 4-7")
+
+
 def test_parity_navier_stokes_3d_box():
     from tests.helpers import ns3_box
     _run_pair(ns3_box(level=3), 2, "navier-stokes 3d 8^3")
