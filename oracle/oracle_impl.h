@@ -7,6 +7,11 @@
  *   OMODEL  O_EULER, O_NAVIER_STOKES or O_MULTI
  *   OPFX(n) the exported-symbol prefix
  *
+ * Parity unpinned: the coarse/fine flux ownership and the lost flux of a
+ * coarse leaf on a finer time level (flux_phase, impl/flux.hpp:58-59) are
+ * checked only against the reference's code by reading; every other phase
+ * has a closed-form or known-answer test under tests/.
+ *
  * Every function names the reference file:line it restates.  Arithmetic is
  * written in the reference's evaluation order and built with
  * -ffp-contract=off so that the HIP path, written to the same order, agrees
@@ -1750,6 +1755,26 @@ void OPFX(t_compute_rhs)(const o_params *P, const double *p, const double *slope
 #endif
     compute_rhs(P, &c);
     for (int i = 0; i < NE; ++i) rhs[i] = c.rhs[i];
+}
+/* Model::flux of one face between two cells, each given as a pack
+ * [p (NE), slope (OD x NE), rhs (NE), t, gradient (OD x NGX), k (NPX)] */
+static void t_unpack(const double *q, o_cell *c) {
+    memset(c, 0, sizeof *c);
+    for (int i = 0; i < NE; ++i) c->p[i] = *q++;
+    for (int d = 0; d < OD; ++d)
+        for (int i = 0; i < NE; ++i) c->slope[d][i] = *q++;
+    for (int i = 0; i < NE; ++i) c->rhs[i] = *q++;
+    c->t = *q++;
+    for (int d = 0; d < OD; ++d)
+        for (int i = 0; i < NGX; ++i) c->gradient[d][i] = *q++;
+    for (int i = 0; i < NPX; ++i) c->k[i] = *q++;
+}
+void OPFX(t_model_flux)(const o_params *P, const double *cn, double cnc, const double *cp, double cpc, double fc,
+                        double area, double t0, double t1, int dim, double *out) {
+    o_cell a, b;
+    t_unpack(cn, &a);
+    t_unpack(cp, &b);
+    model_flux(P, &a, cnc, &b, cpc, fc, area, t0, t1, dim, out);
 }
 int OPFX(t_sizes)(int *out) {
     out[0] = OD;
