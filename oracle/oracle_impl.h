@@ -7,10 +7,10 @@
  *   OMODEL  O_EULER, O_NAVIER_STOKES or O_MULTI
  *   OPFX(n) the exported-symbol prefix
  *
- * Parity unpinned: the coarse/fine flux ownership and the lost flux of a
- * coarse leaf on a finer time level (flux_phase, impl/flux.hpp:58-59) are
- * checked only against the reference's code by reading; every other phase
- * has a closed-form or known-answer test under tests/.
+ * Parity unpinned: the lost flux of a coarse leaf on a finer time level
+ * (flux_phase, impl/flux.hpp:58-59) is checked only against the reference's
+ * code by reading; every other phase has a closed-form, conservation or
+ * known-answer test under tests/.
  *
  * Every function names the reference file:line it restates.  Arithmetic is
  * written in the reference's evaluation order and built with
