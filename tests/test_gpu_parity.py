@@ -89,6 +89,14 @@ def test_parity_multi3d_shock_drop_levels_4_7():
     _run_pair(workloads.shock_drop(base=4, extra=3), 3, "multi3d shock-drop levels 4-7")
 
 
+def test_parity_euler_static_amr_zone():
+    # the conservation case of the oracle: a refinement zone one level above
+    # the base, faces between tree levels and between time levels
+    from tests.test_oracle_amr_conservation import _case
+    for dim in (2, 3):
+        _run_pair(_case(dim), 3, f"euler{dim}d static AMR zone")
+
+
 def test_parity_navier_stokes_3d_box():
     from tests.helpers import ns3_box
     _run_pair(ns3_box(level=3), 2, "navier-stokes 3d 8^3")
