@@ -137,22 +137,22 @@ __global__ void k_softmax_ce(const float *__restrict__ z, int ldz, int nout, con
 
 // db[o] = sum over rows of dz[r][o] (fixed order)
 // out[o] = sum over the rows of dz[r*ld + o].  A block takes 16 columns
-// with 16 lanes each: lane y adds rows y, y+16, ... (coalesced across the
-// columns) and the first lane of a column adds the 16 partial sums in lane
+// with 64 lanes each: lane y adds rows y, y+64, ... (coalesced across the
+// columns) and the first lane of a column adds the 64 partial sums in lane
 // order, so the result does not depend on timing.
-__global__ void __launch_bounds__(256) k_colsum(const float *__restrict__ dz, int ld, int rows, int ncol,
-                                                float *__restrict__ out) {
-    __shared__ float part[16][17];
+__global__ void __launch_bounds__(1024) k_colsum(const float *__restrict__ dz, int ld, int rows, int ncol,
+                                                 float *__restrict__ out) {
+    __shared__ float part[64][17];
     const int x = threadIdx.x & 15, y = threadIdx.x >> 4;
     const int o = blockIdx.x * 16 + x;
     float s = 0.0f;
     if (o < ncol)
-        for (int r = y; r < rows; r += 16) s += dz[(int64_t) r * ld + o];
+        for (int r = y; r < rows; r += 64) s += dz[(int64_t) r * ld + o];
     part[y][x] = s;
     __syncthreads();
     if (y == 0 && o < ncol) {
         float t = part[0][x];
-        for (int k = 1; k < 16; ++k) t += part[k][x];
+        for (int k = 1; k < 64; ++k) t += part[k][x];
         out[o] = t;
     }
 }
@@ -311,7 +311,7 @@ int backward(dacti_mlp *m, int rows, int nb) {
         // gW_l(o, i) = sum_r dZ(r, o) A_prev(r, i)
         int rc = gemm(m, m->dZ16, 1, ldz, aprev, ldp, 1, m->op[l], m->ip[l], rows, m->gW[l], m->ip[l], nullptr, nullptr, 0);
         if (rc) return rc;
-        k_colsum<<<(ldz + 15) / 16, 256, 0, m->s>>>(m->dZ, ldz, rows, ldz, m->gb[l]);
+        k_colsum<<<(ldz + 15) / 16, 1024, 0, m->s>>>(m->dZ, ldz, rows, ldz, m->gb[l]);
         MCHK(m, hipGetLastError());
         if (l > 0) {
             // dA(r, i) = sum_o dZ(r, o) W(o, i); then the ReLU mask of layer l-1
