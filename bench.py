@@ -341,7 +341,8 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None,
-                    help="timed global steps (default 20; 48 for learned; 3 for the shock-drop, whose state turns NaN in step 6)")
+                    help="timed global steps (default 20; 48 for learned, 4 for batched; 3 for the shock-drop, whose state "
+                         "turns NaN in step 6)")
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--level", type=int, default=8)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -368,7 +369,7 @@ def main():
         # the 3D shock-drop at levels 5-9 ends in a NaN state (force_break)
         # in global step 6 (at levels 4-8 the oracle breaks down in the same
         # step as the GPU, their states bitwise equal before it): time steps 3-5
-        args.steps = {"drop": 3, "learned": 48}.get(args.workload, 20)
+        args.steps = {"drop": 3, "learned": 48, "batched": 4}.get(args.workload, 20)
     if args.gpus != 1:
         print(f"bench.py: this engine runs on one GPU; --gpus {args.gpus} is not supported", file=sys.stderr)
         sys.exit(2)
