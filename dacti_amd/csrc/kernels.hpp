@@ -1010,7 +1010,7 @@ __device__ __forceinline__ void sweep_body(const Dev &d, const MConst &c, const 
         if (pass == 0) {
             // rhs[i] gains the dim-j terms in the reference's (i, j, k) order
             rhs_add_dim<MODEL, D>(c, pc, sl, dim, kap, a_ip_in, a_in_ip, a_in_ia, rhs);
-            for (int e = 0; e < NE; ++e) d.slope[(dim * NE + e) * C + y] = sl[e];
+            for (int e = 0; e < NE; ++e) __builtin_nontemporal_store(sl[e], &d.slope[(dim * NE + e) * C + y]);
             if constexpr (S::NG > 0) {
                 double gp[S::NG], gn[S::NG];
                 grad_input<MODEL, D>(c, pp, gp);
@@ -1031,7 +1031,7 @@ __device__ __forceinline__ void sweep_body(const Dev &d, const MConst &c, const 
         err += sq;
     }
     if (pass == 0)
-        for (int e = 0; e < NE; ++e) d.rhs[e * C + y] = rhs[e];
+        for (int e = 0; e < NE; ++e) __builtin_nontemporal_store(rhs[e], &d.rhs[e * C + y]);
     bool lowered = false, marked = false;
     if (deferred) {
         d.mark[y] = MK_DEFER;
