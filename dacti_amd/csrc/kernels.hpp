@@ -917,8 +917,8 @@ __device__ inline void lts_decide(const Dev &d, const MConst &c, int32_t y, cons
 // value the error estimator uses, h false for a boundary face.  DIR is a
 // template parameter so the per-side arrays stay in registers.
 template <int MODEL, int D, int DIR>
-__device__ __forceinline__ void sweep_side(const Dev &d, const MConst &c, int32_t y, int dim, const double *pc,
-                                           const double *ctr, double ys, double yt, int ylev, int L, int pass,
+__device__ __forceinline__ void sweep_side(const Dev &d, const MConst &c, int32_t y, int dim, double *pc,
+                                           double *ctr, double ys, double yt, int ylev, int L, int pass,
                                            double *pv, double *qv, double &tv, double &xv, bool &hv,
                                            bool &deferred) {
     using S = Sz<MODEL, D>;
@@ -941,6 +941,11 @@ __device__ __forceinline__ void sweep_side(const Dev &d, const MConst &c, int32_
         for (int e = 0; e < NE; ++e) pv[e] = tmp[e];
         tv = interp_t<D>(d, nb);
         for (int e = 0; e < NE; ++e) qv[e] = pv[e];
+        // the cell's own state and centre are read again after the calls
+        // (same values), so they are not live across them: otherwise their
+        // registers are saved to private memory for every cell, walk or not
+        for (int e = 0; e < NE; ++e) pc[e] = d.p[e * C + y];
+        for (int a = 0; a < D; ++a) ctr[a] = d.center[a * C + y];
     } else {
         for (int e = 0; e < NE; ++e) pv[e] = d.p[e * C + nb];
         tv = d.t[nb];

@@ -184,6 +184,23 @@ __device__ __forceinline__ double pick(const double *a, int k) {
     return v;
 }
 
+// pick for an array the caller keeps in registers: the values pass through
+// an empty asm first, so the select chain cannot be folded back into an
+// indexed load (which would put the whole array in private memory)
+template <int N>
+__device__ __forceinline__ double pick_reg(const double *a, int k) {
+    double v[N];
+#pragma unroll
+    for (int q = 0; q < N; ++q) {
+        v[q] = a[q];
+        asm volatile("" : "+v"(v[q]));
+    }
+    double r = v[0];
+#pragma unroll
+    for (int q = 1; q < N; ++q) r = k == q ? v[q] : r;
+    return r;
+}
+
 // rhs = sum_j A_j(p) slope_j (Hancock time derivative): euler.hpp:86-116,
 // multi.hpp:126-169.  Only the structurally non-zero Jacobian entries are
 // summed, in the reference's (i, j, k) order; the skipped products are
@@ -211,7 +228,7 @@ __device__ inline void rhs_add_dim(const MConst &c, const double *p, const doubl
     constexpr int NE = S::NE, IU = S::IU, IP = S::IP;
     const int in = IU + j;
     // the velocity component j of p and sl, selected without indexing
-    const double p_in = pick<D>(p + IU, j), s_in = pick<D>(sl + IU, j);
+    const double p_in = pick_reg<D>(p + IU, j), s_in = pick_reg<D>(sl + IU, j);
 #pragma unroll
     for (int i = 0; i < NE; ++i) {
         double r = rhs[i];
