@@ -59,19 +59,27 @@ struct Status {
 
 __device__ __forceinline__ bool is_leaf(const Dev &d, int32_t i) { return d.flags[i] & F_LEAF; }
 
+// a[k] for a runtime k < N, as a chain of selects over values passed through
+// an empty asm, so that the chain cannot be folded back into an indexed load
+// (which would keep the array in private memory)
+template <int N>
+__device__ __forceinline__ int pick_i(const int *a, int k) {
+    int w[N];
+#pragma unroll
+    for (int q = 0; q < N; ++q) {
+        w[q] = a[q];
+        asm volatile("" : "+v"(w[q]));
+    }
+    int v = w[0];
+#pragma unroll
+    for (int q = 1; q < N; ++q) v = k == q ? w[q] : v;
+    return v;
+}
+
 // Workgroups are dispatched round-robin over the 8 XCDs (each with its own
 // L2).  Give XCD x a contiguous run of logical blocks, so a run of
 // consecutive list entries (spatial neighbours in Morton order) shares one
 // L2.  A bijection of [0, nb).
-// a[k] for a runtime k < N, as a chain of selects (no indexed private array)
-template <int N>
-__device__ __forceinline__ int pick_i(const int *a, int k) {
-    int v = a[0];
-#pragma unroll
-    for (int q = 1; q < N; ++q) v = k == q ? a[q] : v;
-    return v;
-}
-
 __device__ __forceinline__ int xcd_block(int b, int nb) {
     const int q = nb >> 3, r = nb & 7;
     const int x = b & 7, k = b >> 3;
@@ -294,8 +302,9 @@ __device__ __forceinline__ void load_side(const Dev &d, int32_t i, int dim, Side
 // navier_stokes.hpp:61-82, multi.hpp:249-302.  fc: face centre along dim;
 // cnc, cpc: cell centres along dim after any periodic shift.
 template <int MODEL, int D>
-__device__ inline void model_flux(const MConst &c, const Side<MODEL, D> &n, double cnc, const Side<MODEL, D> &pz,
-                                  double cpc, double fc, double area, double t0, double t1, int dim, double *out) {
+__device__ __forceinline__ void model_flux(const MConst &c, const Side<MODEL, D> &n, double cnc,
+                                           const Side<MODEL, D> &pz, double cpc, double fc, double area, double t0,
+                                           double t1, int dim, double *out) {
     using S = Sz<MODEL, D>;
     constexpr int NE = S::NE, NF = S::NF, IU = S::IU, IP = S::IP;
     const double dt_n = 0.5 * (t1 + t0) - n.t;
@@ -363,6 +372,24 @@ __device__ inline void model_flux(const MConst &c, const Side<MODEL, D> &n, doub
 // sx is x's side, sn the neighbour's; nbc the neighbour centre along dim.
 // Returns F with the reference's sign convention (x receives -F if dir==1,
 // +F if dir==0; the neighbour the opposite).
+
+// model_flux with the face axis as a constant for the five-equation model:
+// each case is inlined with dim fixed, so the per-axis selections index the
+// state arrays statically (with a runtime axis the compiler keeps them in
+// private memory)
+template <int MODEL, int D>
+__device__ __forceinline__ void model_flux_axis(const MConst &c, const Side<MODEL, D> &n, double cnc,
+                                                const Side<MODEL, D> &pz, double cpc, double fc, double area,
+                                                double t0, double t1, int dim, double *out) {
+    if constexpr (MODEL == MULTI) {
+        if (dim == 0) model_flux<MODEL, D>(c, n, cnc, pz, cpc, fc, area, t0, t1, 0, out);
+        else if (D == 2 || dim == 1) model_flux<MODEL, D>(c, n, cnc, pz, cpc, fc, area, t0, t1, 1, out);
+        else model_flux<MODEL, D>(c, n, cnc, pz, cpc, fc, area, t0, t1, D - 1, out);
+    } else {
+        model_flux<MODEL, D>(c, n, cnc, pz, cpc, fc, area, t0, t1, dim, out);
+    }
+}
+
 template <int MODEL, int D>
 __device__ __forceinline__ void face_flux_sides_body(const MConst &c, const Side<MODEL, D> &sx, double xc, double xs, int xlev,
                                        const Side<MODEL, D> &sn, double nbc, int dim, int dir, double dt,
@@ -712,8 +739,8 @@ __device__ __forceinline__ void flux_simple_body(const Dev &d, const MConst &c, 
                     }
                     const double cy = y_computes ? yc : other;
                     const double cn = y_computes ? other : nbc;
-                    if (dir == 1) model_flux<MODEL, D>(c, sy, cy, sn, cn, fc, area, t0, t1, dim, F);
-                    else model_flux<MODEL, D>(c, sn, cn, sy, cy, fc, area, t0, t1, dim, F);
+                    if (dir == 1) model_flux_axis<MODEL, D>(c, sy, cy, sn, cn, fc, area, t0, t1, dim, F);
+                    else model_flux_axis<MODEL, D>(c, sn, cn, sy, cy, fc, area, t0, t1, dim, F);
                     // y receives -F on its + face and +F on its - face
 #pragma unroll
                     for (int e = 0; e < NF; ++e) cc[e] = dir == 1 ? cc[e] - F[e] : cc[e] + F[e];

@@ -175,7 +175,8 @@ __device__ inline void flux_F(const MConst &c, const double *p, double up, int d
 
 
 // a[k] for a runtime k < N, as a chain of selects (an indexed private array
-// would be placed in scratch memory)
+// would be placed in scratch memory); pick_reg below also keeps the chain
+// from being folded back into an indexed load
 template <int N>
 __device__ __forceinline__ double pick(const double *a, int k) {
     double v = a[0];
