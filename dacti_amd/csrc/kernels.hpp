@@ -405,8 +405,8 @@ __device__ __forceinline__ void face_flux_sides_body(const MConst &c, const Side
             if (xc - xs < -0.5 * c.active_size[dim]) nbc -= c.active_size[dim];
         }
     }
-    if (dir == 1) model_flux<MODEL, D>(c, sx, xc, sn, nbc, fc, area, t0, t1, dim, F);
-    else model_flux<MODEL, D>(c, sn, nbc, sx, xc, fc, area, t0, t1, dim, F);
+    if (dir == 1) model_flux_axis<MODEL, D>(c, sx, xc, sn, nbc, fc, area, t0, t1, dim, F);
+    else model_flux_axis<MODEL, D>(c, sn, nbc, sx, xc, fc, area, t0, t1, dim, F);
 }
 
 template <int MODEL, int D>
