@@ -27,7 +27,7 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per leaf-cell update of all substep kernels on this workload, from
 # two rocprofv3 counter runs (FETCH_SIZE x 2.00, WRITE_SIZE x 1.00, calibrated
 # on a 1 GiB device copy): profiles/leaf_update_euler3d_256_hbm_bytes.txt
-MEASURED_BYTES_PER_UPDATE = 1187.0
+MEASURED_BYTES_PER_UPDATE = 1029.0
 
 
 def cpu_baseline(case, tree, seconds_target=20.0):
