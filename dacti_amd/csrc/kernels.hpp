@@ -834,6 +834,7 @@ __device__ inline void compute_slope_dev(const Dev &d, const MConst &c, int32_t 
     double ts[S::NGX];
     if constexpr (S::NG > 0)
         for (int e = 0; e < S::NG; ++e) ts[e] = d.t_slope[e * C + y];
+#pragma unroll
     for (int dim = 0; dim < D; ++dim) {
         const int32_t in = d.nbr[(2 * dim) * C + y], ip = d.nbr[(2 * dim + 1) * C + y];
         double pn[NE], pp[NE], xn, xp, tn, tp;
@@ -844,8 +845,13 @@ __device__ inline void compute_slope_dev(const Dev &d, const MConst &c, int32_t 
             tn = yt;
         } else {
             if (!(d.flags[in] & F_LEAF)) {
-                interp_p<MODEL, D>(d, in, pn);
+                // the walk writes through a pointer: a temporary keeps pn in
+                // registers, and the own state is read again after the calls
+                double tmp[NE];
+                interp_p<MODEL, D>(d, in, tmp);
+                for (int e = 0; e < NE; ++e) pn[e] = tmp[e];
                 tn = interp_t<D>(d, in);
+                for (int e = 0; e < NE; ++e) pc[e] = d.p[e * C + y];
             } else {
                 for (int e = 0; e < NE; ++e) pn[e] = d.p[e * C + in];
                 tn = d.t[in];
@@ -858,8 +864,11 @@ __device__ inline void compute_slope_dev(const Dev &d, const MConst &c, int32_t 
             tp = yt;
         } else {
             if (!(d.flags[ip] & F_LEAF)) {
-                interp_p<MODEL, D>(d, ip, pp);
+                double tmp[NE];
+                interp_p<MODEL, D>(d, ip, tmp);
+                for (int e = 0; e < NE; ++e) pp[e] = tmp[e];
                 tp = interp_t<D>(d, ip);
+                for (int e = 0; e < NE; ++e) pc[e] = d.p[e * C + y];
             } else {
                 for (int e = 0; e < NE; ++e) pp[e] = d.p[e * C + ip];
                 tp = d.t[ip];
