@@ -101,6 +101,10 @@ SIGNATURES = {
     "dacti_batch_last_error": (C.c_char_p, [C.c_void_p]),
 }
 
+# void fn(void *user, dacti_handle *h, int32_t level, double acti_time, int64_t n_substeps, double dt)
+SUBSTEP_HOOK = C.CFUNCTYPE(None, C.c_void_p, C.c_void_p, C.c_int32, C.c_double, C.c_int64, C.c_double)
+SIGNATURES["dacti_set_substep_hook"] = (C.c_int, [C.c_void_p, SUBSTEP_HOOK, C.c_void_p])
+
 _lib = None
 
 

@@ -369,7 +369,6 @@ struct BOps {
             b->flux_ms += a;
             b->substep_ms += a + c;
         }
-        int top = 0;
         for (int i = 0; i < n; ++i) {
             dacti_handle *h = b->hs[(size_t) i];
             if (changed[(size_t) i])
@@ -390,6 +389,25 @@ struct BOps {
                 if (it.flush) h->pending[L] = 0;
                 h->n_substeps++;
             }
+        }
+        update_top(b);
+        b->n_substeps++;
+        // the caller's per-substep hook (impl/integrate.hpp:64), after the
+        // substep of each instance that took part, with its state on the host
+        // side consistent (the stream is idle here)
+        for (int i = 0; i < n; ++i) {
+            dacti_handle *h = b->hs[(size_t) i];
+            if (part[(size_t) i] && h->hook) h->hook(h->hook_user, h, L, h->acti_time[L], h->n_substeps, h->global_dt / (1 << L));
+        }
+        return 0;
+    }
+
+    // highest level any instance may run: a level with a cluster, a pending
+    // buffer, or at most the instance's max_acti_level
+    static void update_top(dacti_batch *b) {
+        int top = 0;
+        for (int i = 0; i < b->n; ++i) {
+            const dacti_handle *h = b->hs[(size_t) i];
             for (int l = 15; l > top; --l)
                 if (h->counts[l] > 0 || h->pending[l] > 0 || l <= h->max_acti_level) {
                     top = l;
@@ -397,8 +415,6 @@ struct BOps {
                 }
         }
         b->top = top;
-        b->n_substeps++;
-        return 0;
     }
 
     static void tune_begin(dacti_batch *b) {
@@ -470,6 +486,9 @@ struct BOps {
                 }
             }
         }
+        // gc re-binned the levels from the global dt (a leaf dropped by the
+        // raise rule comes back on level 15): the next step must see them
+        update_top(b);
         b->n_global_steps++;
         const auto c1 = std::chrono::steady_clock::now();
         tune_end(b, std::chrono::duration<double, std::milli>(c1 - c0).count());

@@ -767,6 +767,13 @@ int dacti_set_launch(dacti_handle *h, int32_t block, int32_t grid, int32_t graph
     return rc;
 }
 
+int dacti_set_substep_hook(dacti_handle *h, dacti_substep_hook fn, void *user) {
+    if (!h) return DACTI_E_ARG;
+    h->hook = fn;
+    h->hook_user = user;
+    return DACTI_OK;
+}
+
 int dacti_set_remesh(dacti_handle *h, int32_t on) {
     if (!h) return DACTI_E_ARG;
     h->remesh = on != 0;

@@ -88,6 +88,8 @@ struct dacti_handle {
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     struct dacti_batch *solo = nullptr;    // batch of this handle alone
     int remesh = 1;  // 0: refinement marks paused (host-set cadence)
+    dacti_substep_hook hook = nullptr;  // called after every substep (impl/integrate.hpp:64)
+    void *hook_user = nullptr;
     std::string err;
 };
 

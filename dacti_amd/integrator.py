@@ -43,13 +43,36 @@ class Integrator:
             raise RuntimeError(f"{what} failed ({rc}): {self.lib.dacti_last_error(self.h).decode()}")
 
     def integrate(self, substep_hook=None, signal=True):
-        """One global step; returns (global_dt, force_break)."""
+        """One global step; returns (global_dt, force_break).
+
+        substep_hook(integrator, level, acti_time, n_substeps, dt) runs after
+        every LTS substep, as in impl/integrate.hpp:64 (through
+        dacti_set_substep_hook); an exception it raises is re-raised here
+        after the step."""
         dt = C.c_double()
         fb = C.c_int32()
-        self._check(self.lib.dacti_integrate(self.h, int(bool(signal)), C.byref(dt), C.byref(fb)), "integrate")
-        if substep_hook is not None:
-            s = self.stats()
-            substep_hook(self, s.max_acti_level, None, s.n_substeps, dt.value)
+        if substep_hook is None:
+            self._check(self.lib.dacti_integrate(self.h, int(bool(signal)), C.byref(dt), C.byref(fb)), "integrate")
+            return dt.value, bool(fb.value)
+        errors = []
+
+        def tramp(user, h, level, acti_time, n_substeps, dtl):
+            if errors:
+                return
+            try:
+                substep_hook(self, level, acti_time, n_substeps, dtl)
+            except BaseException as e:  # noqa: BLE001 - re-raised below
+                errors.append(e)
+
+        cb = _lib.SUBSTEP_HOOK(tramp)
+        self._check(self.lib.dacti_set_substep_hook(self.h, cb, None), "set_substep_hook")
+        try:
+            rc = self.lib.dacti_integrate(self.h, int(bool(signal)), C.byref(dt), C.byref(fb))
+        finally:
+            self.lib.dacti_set_substep_hook(self.h, _lib.SUBSTEP_HOOK(), None)
+        self._check(rc, "integrate")
+        if errors:
+            raise errors[0]
         return dt.value, bool(fb.value)
 
     def run(self, steps, signal=True):

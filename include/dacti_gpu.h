@@ -159,6 +159,18 @@ int dacti_create(const dacti_case *c, const dacti_tree *t, dacti_handle **out);
  * *force_break is set to 1 when a NaN pressure or density appeared. */
 int dacti_integrate(dacti_handle *h, int32_t signal, double *global_dt, int32_t *force_break);
 
+/* The substep_hook argument of Integrator::integrate (impl/integrate.hpp:64):
+ * when set, dacti_integrate / dacti_run (and a batch stepping the handle)
+ * call fn(user, h, level, acti_time[level], n_substeps, dt_level) after
+ * every LTS substep the handle takes, with the handle's state complete, so
+ * fn may call dacti_export_nodes / dacti_get_stats on h.  fn = NULL (the
+ * default, the reference driver's no-op hook) removes it.  With a hook set
+ * every substep ends on the host (as without one; a graphed whole step is
+ * not possible then). */
+typedef void (*dacti_substep_hook)(void *user, dacti_handle *h, int32_t level, double acti_time, int64_t n_substeps,
+                                   double dt);
+int dacti_set_substep_hook(dacti_handle *h, dacti_substep_hook fn, void *user);
+
 /* `steps` calls of dacti_integrate; *completed = steps done before a break. */
 int dacti_run(dacti_handle *h, int64_t steps, int32_t signal, int64_t *completed);
 

@@ -110,6 +110,9 @@ class Oracle:
             self._fn(nm, rt, [C.c_void_p])
         self._fn("destroy", None, [C.c_void_p])
         self._fn("export_nodes", None, [C.c_void_p] + [C.c_void_p] * 17)
+        self._fn("face_cover", None, [C.c_void_p, C.c_void_p])
+        self._fn("set_no_gc", None, [C.c_void_p, C.c_int])
+        self._fn("gc_now", None, [C.c_void_p])
 
     def _fn(self, name, res, args):
         f = getattr(self.L, self.pfx + name)
@@ -165,6 +168,18 @@ class Oracle:
                  "t", "p", "u", "slope", "rhs", "k", "grad", "meta"]
         self._export_nodes(self.h, *[a[k].ctypes.data for k in order])
         return a
+
+    def face_cover(self):
+        """Time coverage of every face over the last global step (n x 2D)."""
+        out = np.zeros((self.n_nodes, 2 * self.case.dim))
+        self._face_cover(self.h, out.ctypes.data)
+        return out
+
+    def set_no_gc(self, on=True):
+        self._set_no_gc(self.h, int(on))
+
+    def gc_now(self):
+        self._gc_now(self.h)
 
     def close(self):
         if self.h:

@@ -7,10 +7,17 @@
  *   OMODEL  O_EULER, O_NAVIER_STOKES or O_MULTI
  *   OPFX(n) the exported-symbol prefix
  *
- * Parity unpinned: the lost flux of a coarse leaf on a finer time level
- * (flux_phase, impl/flux.hpp:58-59) is checked only against the reference's
- * code by reading; every other phase has a closed-form, conservation or
- * known-answer test under tests/.
+ * Pins (tests/test_oracle_*.py): every function below has a closed-form,
+ * brute-force or invariant test -- model functions against the textbook
+ * equations (Jacobian rows, HLLC states, EOS limits, limiters, viscous
+ * stress), split / unsplit / LTS reassignment / boundary flux through the
+ * t_* hooks at the end of this file, the flux ownership through the face
+ * time coverage (every face covered for exactly the global step, except the
+ * coarse/fine faces of a coarse leaf on a finer time level, which the
+ * reference loses), neighbours and 2:1 flags by brute-force search, and the
+ * schedule, slopes and five-equation sources by invariants of the scheme.
+ * The reference ships no golden vectors, so nothing here is pinned against
+ * reference output.
  *
  * Every function names the reference file:line it restates.  Arithmetic is
  * written in the reference's evaluation order and built with
@@ -91,6 +98,11 @@ typedef struct {
 typedef struct {
     o_cell cell;
     double acc[NFACE][NF]; /* received flux per face (replaces node.flux) */
+    /* test instrumentation: time coverage of each face in the current global
+     * step (sum of t1 - t0 over the fluxes applied to the face, weighted by
+     * the evaluated face's share of this cell's face area), and the same for
+     * the received fluxes still held in acc */
+    double cov[NFACE], acc_cov[NFACE];
     int32_t parent, child_base;
     int32_t nbr[NFACE];
     uint8_t bc[NFACE]; /* 0 = none, else 1 + domain face index */
@@ -111,6 +123,7 @@ typedef struct {
     int64_t cell_updates;
     int64_t counts[O_MAX_LEVEL]; /* cluster sizes */
     int64_t acc_violations; /* received flux found where the read rule says none can be */
+    int no_gc;              /* test switch: never compact (the result must not change) */
     /* scratch */
     double *tot;     /* NF per node, flux total of the current substep */
     int32_t *list;   /* current cluster list */
@@ -764,6 +777,7 @@ static void dom_unsplit(o_state *S, int32_t pidx) {
     pn->leaf = 1;
     pn->error = err;
     memset(pn->acc, 0, sizeof pn->acc);
+    memset(pn->acc_cov, 0, sizeof pn->acc_cov);
     pn->creation_time = c.t;
     pn->active = 1;
     pn->acti = -1;
@@ -982,6 +996,7 @@ static void deposit_from_fine(o_state *S, int32_t r, int fr, int L, double dt) {
         /* x's face dir is 1-dir; the receiver gets +F if x's dir==1 */
         for (int i = 0; i < NF; ++i)
             S->tree[r].acc[fr][i] = (1 - dir) == 1 ? S->tree[r].acc[fr][i] + F[i] : S->tree[r].acc[fr][i] - F[i];
+        S->tree[r].acc_cov[fr] += dt * pow_level(S->tree[x].cell.size / S->tree[r].cell.size, OD - 1);
     }
 }
 
@@ -1013,6 +1028,8 @@ static void flux_phase(o_state *S, const int32_t *list, int64_t nl, int L) {
                 double c[NF], F[NF];
                 for (int i = 0; i < NF; ++i) c[i] = ny->acc[f][i];
                 for (int i = 0; i < NF; ++i) ny->acc[f][i] = 0.0;
+                double cov = ny->acc_cov[f];
+                ny->acc_cov[f] = 0.0;
                 int32_t nb = ny->nbr[f];
                 {
                     /* Received flux can only come from a face neighbour that
@@ -1048,6 +1065,7 @@ static void flux_phase(o_state *S, const int32_t *list, int64_t nl, int L) {
                     boundary_flux(&S->P, ny->bc[f] ? ny->bc[f] - 1 : f, &ny->cell, dim, dir, area, ny->cell.t,
                                   ny->cell.t + dt, F);
                     for (int i = 0; i < NF; ++i) c[i] = c[i] - F[i];
+                    cov += dt;
                 } else if (S->tree[nb].leaf) {
                     o_node *nn = &S->tree[nb];
                     int y_computes, nb_computes = 0;
@@ -1060,11 +1078,13 @@ static void flux_phase(o_state *S, const int32_t *list, int64_t nl, int L) {
                     if (y_computes) {
                         owner_face_flux(S, y, dim, dir, dt, F);
                         for (int i = 0; i < NF; ++i) c[i] = dir == 1 ? c[i] - F[i] : c[i] + F[i];
+                        cov += dt;
                         if (nn->cluster != L) {
                             if (nn->tree_level == ny->tree_level) {
                                 int fo = 2 * dim + (1 - dir);
                                 for (int i = 0; i < NF; ++i)
                                     nn->acc[fo][i] = dir == 1 ? nn->acc[fo][i] + F[i] : nn->acc[fo][i] - F[i];
+                                nn->acc_cov[fo] += dt;
                             } else {
                                 int fo = 2 * dim + (1 - dir);
                                 if (first_contributor(S, nb, fo, L) == y) deposit_from_fine(S, nb, fo, L, dt);
@@ -1073,6 +1093,7 @@ static void flux_phase(o_state *S, const int32_t *list, int64_t nl, int L) {
                     } else if (nb_computes) {
                         owner_face_flux(S, nb, dim, 1 - dir, dt, F); /* nb's dir is 1 */
                         for (int i = 0; i < NF; ++i) c[i] = c[i] + F[i];
+                        cov += dt;
                     }
                 } else {
                     int32_t subs[O_MAX_SUB];
@@ -1083,9 +1104,11 @@ static void flux_phase(o_state *S, const int32_t *list, int64_t nl, int L) {
                         if (S->tree[x].nbr[2 * dim + (1 - dir)] != y) continue;
                         owner_face_flux(S, x, dim, 1 - dir, dt, F);
                         for (int i = 0; i < NF; ++i) c[i] = (1 - dir) == 1 ? c[i] + F[i] : c[i] - F[i];
+                        cov += dt * pow_level(S->tree[x].cell.size / ny->cell.size, OD - 1);
                     }
                 }
                 for (int i = 0; i < NF; ++i) tot[i] = tot[i] + c[i];
+                ny->cov[f] += cov;
             }
     }
 }
@@ -1366,6 +1389,30 @@ static double estimate_error(const o_state *S, int32_t idx) {
 #define MK_UNSPLIT_CAND 2
 #define MK_UNSPLIT_CHILD 4
 
+/* LTS reassignment of one entry of acti_clusters[L]:
+ * impl/adaptive_refinement.hpp:264-322.  nd->max_dt is set by the caller. */
+static void lts_reassign_one(const o_params *P, o_node *nd, int L, double dt, int counter) {
+    nd->cfl = dt / nd->max_dt;
+    if (!nd->leaf || nd->garbage) {
+        nd->cluster = -1;
+        return;
+    }
+    if (nd->cfl < 0.5 * P->max_cfl && counter == 1 && L > 0) {
+        nd->cluster = -1;
+        nd->buffer = (int8_t) (L - 1);
+        nd->cfl *= 2;
+    } else if (nd->cfl > P->max_cfl && L < O_MAX_LEVEL - 1) {
+        int raise = (int) (log2(nd->cfl / P->max_cfl) + 1);
+        if (L + raise < O_MAX_LEVEL) {
+            nd->acti = (int8_t) (L + raise);
+            nd->cfl /= 1 << raise;
+            nd->cluster = (int8_t) (L + raise);
+        } else {
+            nd->cluster = -1;
+        }
+    }
+}
+
 /* adaptive_refinement: impl/adaptive_refinement.hpp:6-323 */
 static void adaptive_refinement(o_state *S, int L) {
     double dt = level_dt(S, L);
@@ -1467,27 +1514,8 @@ static void adaptive_refinement(o_state *S, int L) {
 #pragma omp parallel for schedule(guided, 4)
     for (int64_t li = 0; li < nl; ++li) {
         o_node *nd = &S->tree[S->list[li]];
-        double mdt = max_delta_time(P, &nd->cell);
-        nd->max_dt = mdt;
-        nd->cfl = dt / nd->max_dt;
-        if (!nd->leaf || nd->garbage) {
-            nd->cluster = -1;
-            continue;
-        }
-        if (nd->cfl < 0.5 * P->max_cfl && S->acti_counter[L] == 1 && L > 0) {
-            nd->cluster = -1;
-            nd->buffer = (int8_t) (L - 1);
-            nd->cfl *= 2;
-        } else if (nd->cfl > P->max_cfl && L < O_MAX_LEVEL - 1) {
-            int raise = (int) (log2(nd->cfl / P->max_cfl) + 1);
-            if (L + raise < O_MAX_LEVEL) {
-                nd->acti = (int8_t) (L + raise);
-                nd->cfl /= 1 << raise;
-                nd->cluster = (int8_t) (L + raise);
-            } else {
-                nd->cluster = -1;
-            }
-        }
+        nd->max_dt = max_delta_time(P, &nd->cell);
+        lts_reassign_one(P, nd, L, dt, S->acti_counter[L]);
     }
 }
 
@@ -1633,6 +1661,8 @@ void OPFX(destroy)(void *h) {
 /* integrate(): impl/integrate.hpp:11-87.  Returns global_dt. */
 double OPFX(integrate)(void *h, int signal, int *force_break) {
     o_state *S = (o_state *) h;
+    for (int64_t i = 0; i < S->n; ++i)
+        for (int f = 0; f < NFACE; ++f) S->tree[i].cov[f] = 0.0;
     int64_t cnt[O_MAX_LEVEL];
     for (int l = 0; l < O_MAX_LEVEL; ++l) cnt[l] = -1;
     for (int seq = 1; seq <= (1 << (O_MAX_LEVEL - 1)); ++seq) {
@@ -1654,7 +1684,7 @@ double OPFX(integrate)(void *h, int signal, int *force_break) {
     for (int64_t i = 1; i < S->n; ++i)
         if (S->tree[i].cluster >= 0) nact++;
     S->n_global_steps++;
-    if ((double) nact / (double) ntree < 0.2) {
+    if (!S->no_gc && (double) nact / (double) ntree < 0.2) {
         dom_gc_compact(S);
         for (int64_t i = 1; i < S->n; ++i) S->tree[i].cluster = -1;
         assign_all(S);
@@ -1715,6 +1745,141 @@ void OPFX(export_nodes)(void *h, int32_t *parent, int32_t *child_base, int32_t *
         meta[i * 4 + 2] = nd->max_dt;
         meta[i * 4 + 3] = nd->cfl;
     }
+}
+
+
+/* ---- test hooks on single phases (closed-form and brute-force pins) ---- */
+
+/* face time coverage of the last global step, n x NFACE */
+void OPFX(face_cover)(void *h, double *out) {
+    o_state *S = (o_state *) h;
+    for (int64_t i = 0; i < S->n; ++i)
+        for (int f = 0; f < NFACE; ++f) out[i * NFACE + f] = S->tree[i].cov[f];
+}
+void OPFX(set_no_gc)(void *h, int on) { ((o_state *) h)->no_gc = on; }
+/* the compaction integrate() runs when the active fraction is low
+ * (impl/integrate.hpp:78-84), run now */
+void OPFX(gc_now)(void *h) {
+    o_state *S = (o_state *) h;
+    dom_gc_compact(S);
+    for (int64_t i = 1; i < S->n; ++i) S->tree[i].cluster = -1;
+    assign_all(S);
+    recount(S);
+}
+
+/* LTS reassignment of one cell (lts_reassign_one): in: max_dt, level L, dt,
+ * counter, leaf; out[4] = cluster, buffer, acti, cfl (acti_in is the cell's
+ * ACTI level before the decision) */
+void OPFX(t_lts_reassign)(const o_params *P, double max_dt, int L, double dt, int counter, int leaf, int acti_in,
+                          double *out) {
+    o_node nd;
+    memset(&nd, 0, sizeof nd);
+    nd.max_dt = max_dt;
+    nd.leaf = (uint8_t) leaf;
+    nd.acti = (int8_t) acti_in;
+    nd.cluster = (int8_t) L;
+    nd.buffer = -1;
+    lts_reassign_one(P, &nd, L, dt, counter);
+    out[0] = nd.cluster;
+    out[1] = nd.buffer;
+    out[2] = nd.acti;
+    out[3] = nd.cfl;
+}
+
+static o_state *t_scratch_state(const o_params *P, int64_t n) {
+    o_state *S = (o_state *) calloc(1, sizeof(o_state));
+    S->P = *P;
+    ensure_cap(S, n + 16);
+    S->n = n;
+    memset(S->tree, 0, (size_t) S->cap * sizeof(o_node));
+    return S;
+}
+
+/* Domain::split of one leaf given as [p NE, slope OD*NE, k NPX, gradient
+ * OD*NGX, t_slope NGX, center OD, size, t]; children written as
+ * [p NE, u NE, center OD, size, t, slope OD*NE, k NPX, gradient OD*NGX,
+ * t_slope NGX] per child */
+void OPFX(t_split)(const o_params *P, const double *q, double *out) {
+    o_state *S = t_scratch_state(P, 2);
+    o_node *pn = &S->tree[1];
+    pn->leaf = 1;
+    pn->active = 1;
+    for (int i = 0; i < NE; ++i) pn->cell.p[i] = *q++;
+    for (int d = 0; d < OD; ++d)
+        for (int i = 0; i < NE; ++i) pn->cell.slope[d][i] = *q++;
+    for (int i = 0; i < NPX; ++i) pn->cell.k[i] = *q++;
+    for (int d = 0; d < OD; ++d)
+        for (int i = 0; i < NGX; ++i) pn->cell.gradient[d][i] = *q++;
+    for (int i = 0; i < NGX; ++i) pn->cell.t_slope[i] = *q++;
+    for (int d = 0; d < OD; ++d) pn->cell.center[d] = *q++;
+    pn->cell.size = *q++;
+    pn->cell.t = *q++;
+    dom_split(S, 1);
+    int32_t cb = S->tree[1].child_base;
+    for (int ch = 0; ch < NCH; ++ch) {
+        const o_cell *c = &S->tree[cb + ch].cell;
+        for (int i = 0; i < NE; ++i) *out++ = c->p[i];
+        for (int i = 0; i < NE; ++i) *out++ = c->u[i];
+        for (int d = 0; d < OD; ++d) *out++ = c->center[d];
+        *out++ = c->size;
+        *out++ = c->t;
+        for (int d = 0; d < OD; ++d)
+            for (int i = 0; i < NE; ++i) *out++ = c->slope[d][i];
+        for (int i = 0; i < NPX; ++i) *out++ = c->k[i];
+        for (int d = 0; d < OD; ++d)
+            for (int i = 0; i < NGX; ++i) *out++ = c->gradient[d][i];
+        for (int i = 0; i < NGX; ++i) *out++ = c->t_slope[i];
+    }
+    OPFX(destroy)(S);
+}
+
+/* Domain::unsplit of a parent whose NCH leaf children are given as
+ * [u NE, t, k NPX, gradient OD*NGX, t_slope NGX, error] each; pgrad is the
+ * parent's stored gradient (OD*NGX).  out = [u NE, p NE, t, k NPX,
+ * gradient OD*NGX, t_slope NGX, error] of the parent. */
+void OPFX(t_unsplit)(const o_params *P, const double *q, const double *pgrad, double *out) {
+    o_state *S = t_scratch_state(P, 2 + NCH);
+    o_node *pn = &S->tree[1];
+    pn->child_base = 2;
+    pn->active = 1;
+    for (int d = 0; d < OD; ++d)
+        for (int i = 0; i < NGX; ++i) pn->cell.gradient[d][i] = pgrad[d * NGX + i];
+    for (int ch = 0; ch < NCH; ++ch) {
+        o_node *c = &S->tree[2 + ch];
+        c->leaf = 1;
+        c->active = 1;
+        c->parent = 1;
+        for (int i = 0; i < NE; ++i) c->cell.u[i] = *q++;
+        c->cell.t = *q++;
+        for (int i = 0; i < NPX; ++i) c->cell.k[i] = *q++;
+        for (int d = 0; d < OD; ++d)
+            for (int i = 0; i < NGX; ++i) c->cell.gradient[d][i] = *q++;
+        for (int i = 0; i < NGX; ++i) c->cell.t_slope[i] = *q++;
+        c->error = *q++;
+    }
+    dom_unsplit(S, 1);
+    const o_cell *c = &S->tree[1].cell;
+    for (int i = 0; i < NE; ++i) *out++ = c->u[i];
+    for (int i = 0; i < NE; ++i) *out++ = c->p[i];
+    *out++ = c->t;
+    for (int i = 0; i < NPX; ++i) *out++ = c->k[i];
+    for (int d = 0; d < OD; ++d)
+        for (int i = 0; i < NGX; ++i) *out++ = c->gradient[d][i];
+    for (int i = 0; i < NGX; ++i) *out++ = c->t_slope[i];
+    *out++ = S->tree[1].error;
+    OPFX(destroy)(S);
+}
+
+/* boundary_flux of a cell with primitives p (NE), gradient (OD*NGX) and time t */
+void OPFX(t_boundary_flux)(const o_params *P, int face, const double *p, const double *grad, double t, int dim,
+                           int dir, double area, double t0, double t1, double *out) {
+    o_cell c;
+    memset(&c, 0, sizeof c);
+    for (int i = 0; i < NE; ++i) c.p[i] = p[i];
+    for (int d = 0; d < OD; ++d)
+        for (int i = 0; i < NGX; ++i) c.gradient[d][i] = grad[d * NGX + i];
+    c.t = t;
+    boundary_flux(P, face, &c, dim, dir, area, t0, t1, out);
 }
 
 /* Direct access to single-cell model functions, for closed-form tests. */

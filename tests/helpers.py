@@ -16,9 +16,13 @@ def live_leaves(a):
 def compare_pools(g, o, rtol_l1=1e-11):
     """Compare a GPU export with an oracle export.
 
-    Integer topology, LTS levels, clusters and flags must match exactly;
-    conserved variables within rtol_l1 relative L1 over live leaves.
-    Returns a dict describing the comparison (bitwise equality per field)."""
+    Integer topology, LTS levels, clusters and flags must match exactly.
+    Every state field of every live leaf (time, primitives, conserved
+    variables, slopes, rhs, parameters incl. normals and curvature,
+    gradients, metadata error / max_dt / cfl, centre, size) must be bitwise
+    equal: both sides evaluate in the same order.  The relative L1 of u, p
+    and t is reported as well (the 1e-11 bar is implied by bitwise equality).
+    Returns a dict describing the comparison."""
     rep = {}
     assert g["parent"].shape == o["parent"].shape, (g["parent"].shape, o["parent"].shape)
     for k in TOPOLOGY:
@@ -35,6 +39,11 @@ def compare_pools(g, o, rtol_l1=1e-11):
             rel = np.abs(a - b).sum() / (den if den > 0 else 1.0)
             rep[k + "_rel_l1"] = float(rel)
             assert rel <= rtol_l1, f"{k}: relative L1 {rel}"
+        if not eq.all():
+            rows = np.nonzero(~eq.reshape(len(lv), -1).all(axis=1))[0]
+            i = rows[0]
+            raise AssertionError(f"{k} not bitwise equal at {rows.size} leaves, first node {lv[i]}: "
+                                 f"gpu {a[i]} oracle {b[i]}")
     return rep
 
 

@@ -18,9 +18,10 @@ from tests.helpers import amr_sod_case, compare_pools, sod_case, tiny_box
 pytestmark = pytest.mark.gpu
 
 
-def _run_pair(case, steps, label):
+def _run_pair(case, steps, label, tree=None):
     print(f"[parity] {label}: building tree", flush=True)
-    tree = scene.build_tree(case)
+    if tree is None:
+        tree = scene.build_tree(case)
     o = Oracle(case, tree)
     g = Integrator(case, tree)
     reps = []
