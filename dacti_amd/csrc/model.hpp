@@ -251,6 +251,27 @@ __device__ inline void rhs_add_dim(const MConst &c, const double *p, const doubl
         }
         rhs[i] = r;
     }
+    // The reference forms the dense product (every A[i][k] * slope[k],
+    // euler.hpp:105-116), so a structurally zero entry times a NaN or
+    // infinite slope (a limiter's 0/0, slope_limiters.hpp:26-27,41-42)
+    // still reaches the sum; add those products when some slope is not
+    // finite (for finite slopes they are zeros and change nothing).
+    bool fin = true;
+#pragma unroll
+    for (int k = 0; k < NE; ++k) fin = fin && isfinite(sl[k]);
+    if (!fin) {
+#pragma unroll
+        for (int i = 0; i < NE; ++i) {
+            double r = rhs[i];
+#pragma unroll
+            for (int k = 0; k < NE; ++k) {
+                bool nz = (k == i) || (i == 0 && k == in) || (i == in && k == IP) || (i == IP && k == in);
+                if constexpr (MODEL == MULTI) nz = nz || (i == 1 && k == in) || (i == in && k == S::IA);
+                if (!nz) r += 0.0 * sl[k];
+            }
+            rhs[i] = r;
+        }
+    }
 }
 
 template <int MODEL, int D>

@@ -35,8 +35,9 @@ def compare_pools(g, o, rtol_l1=1e-11):
         eq = (a == b) | both_nan
         rep[k + "_bitwise"] = bool(eq.all())
         if k in ("u", "p", "t"):
-            den = np.abs(b).sum()
-            rel = np.abs(a - b).sum() / (den if den > 0 else 1.0)
+            ok = ~both_nan
+            den = np.abs(b[ok]).sum()
+            rel = np.abs(a[ok] - b[ok]).sum() / (den if den > 0 else 1.0)
             rep[k + "_rel_l1"] = float(rel)
             assert rel <= rtol_l1, f"{k}: relative L1 {rel}"
         if not eq.all():
