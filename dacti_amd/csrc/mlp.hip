@@ -13,6 +13,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <random>
 #include <string>
 #include <vector>
@@ -207,6 +208,7 @@ struct dacti_mlp {
     uint16_t *dZ16 = nullptr;
     int32_t *yb = nullptr;
     hipStream_t s = nullptr;
+    int dev = 0;  // device the network lives on
     hipEvent_t e0 = nullptr, e1 = nullptr;
     double train_ms = 0, train_flops = 0, flops = 0;
     int64_t t = 0;  // Adam steps
@@ -418,6 +420,62 @@ int upload_idx(dacti_mlp *m, const std::vector<int32_t> &h, int32_t **d) {
 
 }  // namespace
 
+
+// ------------------------------------------------------------------
+// Inference that allocates nothing (dacti_mlp_forward_noalloc).  Its
+// activations live in a fixed device array, `scratch`, of
+// NOALLOC_SCRATCH_BYTES per GPU, held until the process ends.  In host code
+// the array's name is only a stand-in: the device address is looked up once
+// per device with hipGetSymbolAddress and kept in g_scratch_dev[dev].
+namespace {
+
+constexpr size_t NOALLOC_SCRATCH_BYTES = size_t(32) << 20;
+constexpr int NOALLOC_MAX_DEVICES = 64;
+constexpr int NOALLOC_MAX_ROWS = 8192;
+
+__device__ __align__(256) unsigned char scratch[NOALLOC_SCRATCH_BYTES];
+
+std::mutex g_scratch_mu;
+unsigned char *g_scratch_dev[NOALLOC_MAX_DEVICES] = {};
+
+// device address of this device's scratch (current device = m->dev)
+int scratch_ptr(dacti_mlp *m, unsigned char **out) {
+    if (m->dev < 0 || m->dev >= NOALLOC_MAX_DEVICES) {
+        m->err = "dacti_mlp_forward_noalloc: device id out of range";
+        return -1;
+    }
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_dev[m->dev]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            m->err = std::string("dacti_mlp_forward_noalloc: hipGetSymbolAddress: ") + hipGetErrorString(e);
+            return -2;
+        }
+        g_scratch_dev[m->dev] = (unsigned char *) p;
+    }
+    *out = g_scratch_dev[m->dev];
+    return 0;
+}
+
+// padded BF16 rows: row r < nb <- x[r], zero beyond nb and beyond nin
+__global__ void k_stage_rows(const float *__restrict__ x, int nin, int nb, int rows, uint16_t *__restrict__ xb,
+                             int ldb) {
+    const int r = blockIdx.x;
+    if (r >= rows) return;
+    for (int k = threadIdx.x; k < ldb; k += blockDim.x)
+        xb[(int64_t) r * ldb + k] = (r < nb && k < nin) ? to_bf16(x[(int64_t) r * nin + k]) : 0;
+}
+
+// logits (nb x nout, dense) from the padded last-layer output
+__global__ void k_copy_logits(const float *__restrict__ z, int ldz, int nout, int nb, float *__restrict__ out) {
+    const int64_t i = blockIdx.x * (int64_t) blockDim.x + threadIdx.x;
+    if (i >= (int64_t) nb * nout) return;
+    out[i] = z[(i / nout) * ldz + i % nout];
+}
+
+}  // namespace
+
 extern "C" {
 
 int dacti_mlp_create(int32_t n_in, int32_t n_hidden, int32_t n_layers, int32_t n_out, uint64_t seed,
@@ -433,6 +491,11 @@ int dacti_mlp_create(int32_t n_in, int32_t n_hidden, int32_t n_layers, int32_t n
         return -2;
     }
     dacti_mlp *m = new dacti_mlp();
+    if (hipGetDevice(&m->dev) != hipSuccess) {
+        delete m;
+        g_err = "dacti_mlp_create: no current HIP device";
+        return -2;
+    }
     m->n_in = n_in;
     m->H = n_hidden;
     m->NL = n_layers;
@@ -535,6 +598,65 @@ int dacti_mlp_forward(dacti_mlp *m, const float *x, int64_t n, float *logits) {
     (void) hipFree(dx);
     if (di) (void) hipFree(di);
     return rc;
+}
+
+int dacti_mlp_forward_noalloc(dacti_mlp *m, const float *x_dev, int64_t n, float *logits_dev) {
+    if (!m) return -1;
+    if (n < 0 || n > INT32_MAX || (n > 0 && (!x_dev || !logits_dev))) {
+        m->err = "dacti_mlp_forward_noalloc: invalid arguments";
+        return -1;
+    }
+    if (n == 0) return 0;
+    int cur = -1;
+    if (hipGetDevice(&cur) != hipSuccess || cur != m->dev) {
+        m->err = "dacti_mlp_forward_noalloc: the network's device must be current";
+        return -1;
+    }
+    unsigned char *base = nullptr;
+    int rc = scratch_ptr(m, &base);
+    if (rc) return rc;
+    const int L = m->L;
+    int maxw = 16;
+    for (int l = 0; l < L; ++l) maxw = std::max(maxw, m->op[l]);
+    // per row: BF16 input, FP32 layer output, two BF16 activation buffers
+    const size_t row_bytes = 2 * (size_t) m->ip[0] + 4 * (size_t) maxw + 2 * 2 * (size_t) maxw;
+    int64_t cap = (int64_t) (NOALLOC_SCRATCH_BYTES / (row_bytes + 64)) / 16 * 16;
+    if (cap < 16) {
+        m->err = "dacti_mlp_forward_noalloc: the network is too wide for its " +
+                 std::to_string(NOALLOC_SCRATCH_BYTES >> 20) + " MiB scratch (" + std::to_string(row_bytes) +
+                 " bytes per row, 16 rows needed)";
+        return -1;
+    }
+    cap = std::min<int64_t>(cap, NOALLOC_MAX_ROWS);
+    auto align = [](size_t b) { return (b + 255) / 256 * 256; };
+    uint16_t *X16 = (uint16_t *) base;
+    float *Z = (float *) (base + align(2 * (size_t) cap * m->ip[0]));
+    uint16_t *Aping = (uint16_t *) ((unsigned char *) Z + align(4 * (size_t) cap * maxw));
+    uint16_t *Apong = (uint16_t *) ((unsigned char *) Aping + align(2 * (size_t) cap * maxw));
+    if ((unsigned char *) Apong + 2 * (size_t) cap * maxw > base + NOALLOC_SCRATCH_BYTES) {
+        m->err = "dacti_mlp_forward_noalloc: internal scratch layout overflow";
+        return -1;
+    }
+    for (int64_t s0 = 0; s0 < n; s0 += cap) {
+        const int nb = (int) std::min<int64_t>(cap, n - s0), rows = pad16(nb);
+        k_stage_rows<<<(unsigned) rows, 64, 0, m->s>>>(x_dev + s0 * m->n_in, m->n_in, nb, rows, X16, m->ip[0]);
+        const uint16_t *a = X16;
+        int lda = m->ip[0];
+        for (int l = 0; l < L; ++l) {
+            uint16_t *cb = (l % 2 == 0) ? Aping : Apong;
+            const int tiles = (rows / 16) * (m->op[l] / 16);
+            k_gemm<<<nblk((int64_t) tiles * 64), 256, 0, m->s>>>(a, lda, 1, m->W16[l], 1, m->ip[l], rows, m->op[l],
+                                                              m->ip[l], Z, m->op[l], cb, m->b[l],
+                                                              l == L - 1 ? 0 : 1, 1);
+            a = cb;
+            lda = m->op[l];
+        }
+        k_copy_logits<<<nblk((int64_t) nb * m->n_out), 256, 0, m->s>>>(Z, m->op[L - 1], m->n_out, nb,
+                                                                         logits_dev + s0 * m->n_out);
+        MCHK(m, hipGetLastError());
+    }
+    MCHK(m, hipStreamSynchronize(m->s));
+    return 0;
 }
 
 int dacti_mlp_loss_grad(dacti_mlp *m, const float *x, const int32_t *y, int64_t n, float *loss, float *grad) {

@@ -19,6 +19,7 @@ SIGNATURES = {
     "dacti_mlp_get_params": (C.c_int, [C.c_void_p, _F]),
     "dacti_mlp_set_params": (C.c_int, [C.c_void_p, _F]),
     "dacti_mlp_forward": (C.c_int, [C.c_void_p, _F, C.c_int64, _F]),
+    "dacti_mlp_forward_noalloc": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p]),
     "dacti_mlp_loss_grad": (C.c_int, [C.c_void_p, _F, _I, C.c_int64, _F, _F]),
     "dacti_mlp_train": (C.c_int, [C.c_void_p, _F, _I, C.c_int64, C.c_int32, C.c_int32, C.c_float, C.c_uint64, _F]),
     "dacti_mlp_train_stats": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
@@ -81,6 +82,13 @@ class MLP:
         out = np.empty((n, self.n_out), np.float32)
         self._check(self.lib.dacti_mlp_forward(self.h, x.ctypes.data_as(_F), n, out.ctypes.data_as(_F)), "forward")
         return out
+
+    def forward_noalloc(self, x_dev, n, logits_dev):
+        """forward() on device arrays (addresses as ints) without any
+        allocation: x_dev holds n x n_in FP32 values, logits_dev receives
+        n x n_out FP32 values."""
+        self._check(self.lib.dacti_mlp_forward_noalloc(self.h, C.c_void_p(x_dev), n, C.c_void_p(logits_dev)),
+                    "forward_noalloc")
 
     def predict(self, x):
         return self.forward(x).argmax(axis=1)

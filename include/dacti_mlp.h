@@ -38,6 +38,19 @@ int dacti_mlp_set_params(dacti_mlp *m, const float *params);
 /* logits (n x n_out) of n samples x (n x n_in) */
 int dacti_mlp_forward(dacti_mlp *m, const float *x, int64_t n, float *logits);
 
+/* dacti_mlp_forward that allocates nothing, for callers that cannot call
+ * hipMalloc (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid allocation).  No counterpart in the reference.  x_dev (n x
+ * n_in) and logits_dev (n x n_out) are FP32 device arrays on the network's
+ * device, which must be current; the activations live in a fixed 32 MiB
+ * device scratch array per GPU (held until the process ends), processed in
+ * chunks of up to 8192 rows.  Results equal dacti_mlp_forward's bit for
+ * bit.  Calls that overlap on one GPU share that scratch and race: issue
+ * them from one thread.  Synchronous (returns when the logits are written).
+ * A network whose rows do not fit 16 at a time in the scratch is rejected
+ * with an error. */
+int dacti_mlp_forward_noalloc(dacti_mlp *m, const float *x_dev, int64_t n, float *logits_dev);
+
 /* mean cross-entropy of n labelled samples and its gradient with respect to
  * the parameters (n_params values; may be NULL) */
 int dacti_mlp_loss_grad(dacti_mlp *m, const float *x, const int32_t *y, int64_t n, float *loss, float *grad);
