@@ -132,7 +132,7 @@ struct BOps {
         for (int i = 0; i < b->n; ++i) {
             dacti_handle *h = b->hs[(size_t) i];
             b->h_items[i].d = h->d;
-            b->h_items[i].d.err = &b->d_st[i].dev_error;
+            b->h_items[i].d.err = (GMEM int32_t *) &b->d_st[i].dev_error;
             b->h_items[i].n = h->n;
             b->h_items[i].list = h->d_list;
             b->h_items[i].rest = h->d_list2;

@@ -20,23 +20,32 @@ enum : uint8_t {
 };
 enum : uint8_t { MK_SPLIT = 1, MK_UNSPLIT_CAND = 2, MK_UNSPLIT_CHILD = 4, MK_READY = 8, MK_DEFER = 16, MK_RA_DONE = 32 };
 
+// Pool arrays are global-memory pointers (address space 1): loads through
+// them are global_load instructions, not flat ones (a flat load also counts
+// against the scalar-load counter, so every wait for a scalar load would
+// drain the vector loads in flight).
+#if defined(__HIP_DEVICE_COMPILE__)
+#define GMEM __attribute__((address_space(1)))
+#else
+#define GMEM
+#endif
 struct Dev {
     int64_t C;
-    int32_t *parent, *child_base, *nbr;     // nbr[f*C + i]
-    int8_t *level, *acti, *cluster, *buffer;
-    uint8_t *flags, *bc, *mark;             // bc[f*C + i]
-    int8_t *nextc;                          // cluster after this substep's LTS decision
-    double *center;                         // center[d*C + i]
-    double *size, *t;
-    double *p, *u, *rhs, *tot;              // [e*C + i]
-    double *k, *t_slope;                    // [e*C + i]
-    double *slope;                          // [(d*NE + e)*C + i]
-    double *grad;                           // [(d*NGX + e)*C + i]
-    double *acc;                            // [(f*NF + e)*C + i]
-    double *dep;                            // [(d*NF + e)*C + i] flux of a fine cell's face toward a coarser receiver
-    double *error, *ctime, *max_dt, *cfl;
-    double *scratch;                        // C doubles (smoothing / reductions)
-    int32_t *err;                           // device error word (subtree deeper than supported)
+    GMEM int32_t *parent, *child_base, *nbr;  // nbr[f*C + i]
+    GMEM int8_t *level, *acti, *cluster, *buffer;
+    GMEM uint8_t *flags, *bc, *mark;          // bc[f*C + i]
+    GMEM int8_t *nextc;                       // cluster after this substep's LTS decision
+    GMEM double *center;                      // center[d*C + i]
+    GMEM double *size, *t;
+    GMEM double *p, *u, *rhs, *tot;           // [e*C + i]
+    GMEM double *k, *t_slope;                 // [e*C + i]
+    GMEM double *slope;                       // [(d*NE + e)*C + i]
+    GMEM double *grad;                        // [(d*NGX + e)*C + i]
+    GMEM double *acc;                         // [(f*NF + e)*C + i]
+    GMEM double *dep;                         // [(d*NF + e)*C + i] flux of a fine cell's face toward a coarser receiver
+    GMEM double *error, *ctime, *max_dt, *cfl;
+    GMEM double *scratch;                     // C doubles (smoothing / reductions)
+    GMEM int32_t *err;                        // device error word (subtree deeper than supported)
 };
 
 // device status block read back by the host after a substep phase
@@ -87,9 +96,9 @@ __device__ __forceinline__ int xcd_block(int b, int nb) {
 }
 
 // byte-wide atomic OR through the enclosing aligned 32-bit word
-__device__ __forceinline__ void atomic_or_u8(uint8_t *addr, uint8_t v) {
-    const uintptr_t a = reinterpret_cast<uintptr_t>(addr);
-    unsigned int *w = reinterpret_cast<unsigned int *>(a & ~uintptr_t(3));
+__device__ __forceinline__ void atomic_or_u8(GMEM uint8_t *addr, uint8_t v) {
+    const uintptr_t a = (uintptr_t) addr;
+    unsigned int *w = (unsigned int *) (a & ~uintptr_t(3));
     atomicOr(w, (unsigned int) v << (8 * (a & 3)));
 }
 
@@ -130,7 +139,7 @@ __device__ inline int32_t dom_find(const Dev &d, const MConst &c, double *p, int
 // up to 16 levels in private memory (a non-inlined, rarely taken path) and
 // adds the child means in the reference's order.
 template <int NV, typename Leaf>
-__device__ __forceinline__ void interp_walk(const uint8_t *flags, const int32_t *child_base, int32_t *err,
+__device__ __forceinline__ void interp_walk(const GMEM uint8_t *flags, const GMEM int32_t *child_base, GMEM int32_t *err,
                                             int32_t idx, double *out, int nch, Leaf &&leaf) {
     if (flags[idx] & F_LEAF) {
         leaf(idx, out);
@@ -174,7 +183,7 @@ __device__ __forceinline__ void interp_walk(const uint8_t *flags, const int32_t 
             continue;
         }
         if (dp + 1 == MAXD) {  // deeper than any tree of 16 levels
-            atomicOr(err, 1);
+            atomicOr((int32_t *) err, 1);
             pos[dp]++;
             continue;
         }
@@ -189,16 +198,16 @@ __device__ __forceinline__ void interp_walk(const uint8_t *flags, const int32_t 
 // The non-inlined walks take plain pointers: a reference to the kernel's
 // by-value Dev argument would make every thread copy it to scratch.
 template <int NE, int NCH>
-__device__ __noinline__ void interp_vec(const uint8_t *flags, const int32_t *child_base, int32_t *err,
-                                        const double *field, int64_t C, int32_t idx, double *out) {
+__device__ __noinline__ void interp_vec(const GMEM uint8_t *flags, const GMEM int32_t *child_base, GMEM int32_t *err,
+                                        const GMEM double *field, int64_t C, int32_t idx, double *out) {
     interp_walk<NE>(flags, child_base, err, idx, out, NCH, [&](int32_t i, double *o) {
         for (int e = 0; e < NE; ++e) o[e] = field[e * C + i];
     });
 }
 
 template <int NCH>
-__device__ __noinline__ double interp_scalar(const uint8_t *flags, const int32_t *child_base, int32_t *err,
-                                             const double *field, int32_t idx) {
+__device__ __noinline__ double interp_scalar(const GMEM uint8_t *flags, const GMEM int32_t *child_base, GMEM int32_t *err,
+                                             const GMEM double *field, int32_t idx) {
     double o[1];
     interp_walk<1>(flags, child_base, err, idx, o, NCH, [&](int32_t i, double *r) { r[0] = field[i]; });
     return o[0];
@@ -252,7 +261,7 @@ __device__ inline void for_sub_face_leaves(const Dev &d, int32_t root, int dim, 
             for (int c = 0; c < (1 << D); ++c) {
                 if (((c >> dim) & 1) != side) continue;
                 const int32_t i3 = c3 + c;
-                if (!(d.flags[i3] & F_LEAF)) atomicOr(d.err, 1);
+                if (!(d.flags[i3] & F_LEAF)) atomicOr((int32_t *) d.err, 1);
                 visit_leaf(i3);
             }
         }
@@ -301,26 +310,30 @@ __device__ __forceinline__ void load_side(const Dev &d, int32_t i, int dim, Side
 // Model::flux of the face (cn | cp): euler.hpp:156-200,
 // navier_stokes.hpp:61-82, multi.hpp:249-302.  fc: face centre along dim;
 // cnc, cpc: cell centres along dim after any periodic shift.
+// one side's MUSCL-Hancock state at the face: p + slope dx - rhs (tm - t),
+// tm = (t0 + t1) / 2 (euler.hpp:168-181)
 template <int MODEL, int D>
-__device__ __forceinline__ void model_flux(const MConst &c, const Side<MODEL, D> &n, double cnc,
-                                           const Side<MODEL, D> &pz, double cpc, double fc, double area, double t0,
-                                           double t1, int dim, double *out) {
+__device__ __forceinline__ void face_state(const double *p, const double *s, const double *r, double t, double dx,
+                                           double tm, double *q) {
+    constexpr int NE = Sz<MODEL, D>::NE;
+    const double dta = tm - t;
+#pragma unroll
+    for (int e = 0; e < NE; ++e) q[e] = p[e] + s[e] * dx;
+#pragma unroll
+    for (int e = 0; e < NE; ++e) q[e] = q[e] - r[e] * dta;
+}
+
+// The face flux from the two face states pn | pp (euler.hpp:182-199,
+// navier_stokes.hpp:61-82, multi.hpp:262-302).  The cell sides n and pz
+// enter only through means (curvature, normal, gradients, velocity), so
+// their order does not matter.
+template <int MODEL, int D>
+__device__ __forceinline__ void riemann_flux(const MConst &c, const double *pn, const double *pp,
+                                             const Side<MODEL, D> &n, const Side<MODEL, D> &pz, double area, double t0,
+                                             double t1, int dim, double *out) {
     using S = Sz<MODEL, D>;
     constexpr int NE = S::NE, NF = S::NF, IU = S::IU, IP = S::IP;
-    const double dt_n = 0.5 * (t1 + t0) - n.t;
-    const double dt_p = 0.5 * (t1 + t0) - pz.t;
-    const double dxn = fc - cnc, dxp = fc - cpc;
-    double pn[NE], pp[NE], pm[NE], F[NF];
-#pragma unroll
-    for (int e = 0; e < NE; ++e) {
-        pn[e] = n.p[e] + n.s[e] * dxn;
-        pp[e] = pz.p[e] + pz.s[e] * dxp;
-    }
-#pragma unroll
-    for (int e = 0; e < NE; ++e) {
-        pn[e] = pn[e] - n.r[e] * dt_n;
-        pp[e] = pp[e] - pz.r[e] * dt_p;
-    }
+    double pm[NE], F[NF];
     if constexpr (MODEL == MULTI) {
         constexpr int IA = S::IA;
         const double ka = 0.5 * (n.kap + pz.kap);
@@ -366,6 +379,18 @@ __device__ __forceinline__ void model_flux(const MConst &c, const Side<MODEL, D>
             out[IU + D] -= c.ns_lambda * du[dim][S::IDT] * area * (t1 - t0);
         }
     }
+}
+
+template <int MODEL, int D>
+__device__ __forceinline__ void model_flux(const MConst &c, const Side<MODEL, D> &n, double cnc,
+                                           const Side<MODEL, D> &pz, double cpc, double fc, double area, double t0,
+                                           double t1, int dim, double *out) {
+    constexpr int NE = Sz<MODEL, D>::NE;
+    const double tm = 0.5 * (t1 + t0);
+    double pn[NE], pp[NE];
+    face_state<MODEL, D>(n.p, n.s, n.r, n.t, fc - cnc, tm, pn);
+    face_state<MODEL, D>(pz.p, pz.s, pz.r, pz.t, fc - cpc, tm, pp);
+    riemann_flux<MODEL, D>(c, pn, pp, n, pz, area, t0, t1, dim, out);
 }
 
 // Flux of owner x's face (dim, dir) from the two loaded sides: impl/flux.hpp:32-91.
@@ -562,7 +587,7 @@ __device__ void flux_cell(const Dev &d, const MConst &c, int32_t y, int L, doubl
                         if (nlev == ylev) {
 #pragma unroll
                             for (int e = 0; e < NF; ++e) {
-                                double &a = d.acc[(fo * NF + e) * C + nb];
+                                GMEM double &a = d.acc[(fo * NF + e) * C + nb];
                                 a = dir == 1 ? a + F[e] : a - F[e];
                             }
                         } else {
@@ -630,7 +655,7 @@ __device__ void deposit_coarse(const Dev &d, int32_t y, int L) {
         for_sub_face_leaves<D>(d, rn, dim, dir, [&](int32_t x) {
             if (d.cluster[x] == L && d.nbr[f * C + x] == nb) {
                 for (int e = 0; e < NF; ++e) {
-                    double &a = d.acc[(fo * NF + e) * C + nb];
+                    GMEM double &a = d.acc[(fo * NF + e) * C + nb];
                     const double g = d.dep[(dim * NF + e) * C + x];
                     a = dir == 1 ? a + g : a - g;
                 }
@@ -748,7 +773,7 @@ __device__ __forceinline__ void flux_simple_body(const Dev &d, const MConst &c, 
                         const int fo = 2 * dim + (1 - dir);
 #pragma unroll
                         for (int e = 0; e < NF; ++e) {
-                            double &a = d.acc[(fo * NF + e) * C + nb];
+                            GMEM double &a = d.acc[(fo * NF + e) * C + nb];
                             a = dir == 1 ? a + F[e] : a - F[e];
                         }
                     }

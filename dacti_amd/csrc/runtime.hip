@@ -95,10 +95,10 @@ int alloc_pool(dacti_handle *h, int64_t C, Dev *old, int64_t n) {
     };
     HIPCHK(h, hipcub::DeviceSelect::If(nullptr, b1, it, h->d_list, h->d_nsel, (int) C, Always{}, h->stream));
     HIPCHK(h, hipcub::DeviceScan::ExclusiveSum(nullptr, b2, h->d_rn, h->d_off, (int) C, h->stream));
-    HIPCHK(h, hipcub::DeviceReduce::Max(nullptr, b3, nd.scratch, (double *) nd.tot, (int) C, h->stream));
+    HIPCHK(h, hipcub::DeviceReduce::Max(nullptr, b3, (double *) nd.scratch, (double *) nd.tot, (int) C, h->stream));
     h->tmp_bytes = std::max(b1, std::max(b2, b3)) + 1024;
     HIPCHK(h, hipMalloc(&h->d_tmp, h->tmp_bytes));
-    nd.err = &h->d_status_home->dev_error;
+    nd.err = (GMEM int32_t *) &h->d_status_home->dev_error;
     h->d = nd;
     return DACTI_OK;
 }
@@ -199,7 +199,7 @@ int gc(dacti_handle *h) {
     HIPCHK(h, hipMemsetAsync(nd.cluster + n_new, 0xff, (size_t) (nd.C - n_new), h->stream));
     HIPCHK(h, hipMemsetAsync(nd.buffer + n_new, 0xff, (size_t) (nd.C - n_new), h->stream));
     HIPCHK(h, hipMemsetAsync(nd.acti + n_new, 0xff, (size_t) (nd.C - n_new), h->stream));
-    nd.err = &h->d_status_home->dev_error;
+    nd.err = (GMEM int32_t *) &h->d_status_home->dev_error;
     k_gc_remap<<<blocks(n_new), 256, 0, h->stream>>>(nd, n_new, h->sz.NFACE, keep, o2n);
     HIPCHK(h, hipGetLastError());
     HIPCHK(h, hipStreamSynchronize(h->stream));

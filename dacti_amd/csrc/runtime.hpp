@@ -176,7 +176,7 @@ inline Item base_item(dacti_batch *b, int i) {
     dacti_handle *h = b->hs[(size_t) i];
     Item it{};
     it.d = h->d;
-    it.d.err = &b->d_st[i].dev_error;
+    it.d.err = (GMEM int32_t *) &b->d_st[i].dev_error;
     it.list = h->d_list;
     it.rest = h->d_list2;
     it.st = b->d_st + i;
@@ -202,8 +202,8 @@ struct Ops {
         k_init_cells<MODEL, D><<<blocks(d.C), 256, 0, h->stream>>>(d, h->mc, h->n);
         HIPCHK(h, hipGetLastError());
         size_t bytes = h->tmp_bytes;
-        double *d_max = d.tot;  // free until the first substep
-        HIPCHK(h, hipcub::DeviceReduce::Max(h->d_tmp, bytes, d.scratch, d_max, (int) d.C, h->stream));
+        double *d_max = (double *) d.tot;  // free until the first substep
+        HIPCHK(h, hipcub::DeviceReduce::Max(h->d_tmp, bytes, (double *) d.scratch, d_max, (int) d.C, h->stream));
         double gmax = 0;
         HIPCHK(h, hipMemcpyAsync(&gmax, d_max, sizeof(double), hipMemcpyDeviceToHost, h->stream));
         HIPCHK(h, hipStreamSynchronize(h->stream));
