@@ -80,6 +80,7 @@ SIGNATURES = {
     "dacti_n_nodes": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64)]),
     "dacti_export_nodes": (C.c_int, [C.c_void_p, C.POINTER(NodeViewStruct)]),
     "dacti_get_stats": (C.c_int, [C.c_void_p, C.POINTER(StatsStruct)]),
+    "dacti_get_schedule": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int32)]),
     "dacti_set_timing": (C.c_int, [C.c_void_p, C.c_int32]),
     "dacti_set_launch": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32]),
     "dacti_set_remesh": (C.c_int, [C.c_void_p, C.c_int32]),

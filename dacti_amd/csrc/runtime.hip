@@ -817,6 +817,15 @@ int dacti_get_stats(dacti_handle *h, dacti_stats *s) {
     return DACTI_OK;
 }
 
+int dacti_get_schedule(dacti_handle *h, double *acti_time, int32_t *acti_counter) {
+    if (!h) return DACTI_E_ARG;
+    for (int l = 0; l < DACTI_MAX_LEVEL; ++l) {
+        if (acti_time) acti_time[l] = h->acti_time[l];
+        if (acti_counter) acti_counter[l] = h->acti_counter[l];
+    }
+    return DACTI_OK;
+}
+
 int dacti_set_timing(dacti_handle *h, int32_t on) {
     if (!h) return DACTI_E_ARG;
     h->timing = on != 0;

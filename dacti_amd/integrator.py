@@ -114,6 +114,13 @@ class Integrator:
         self._check(self.lib.dacti_get_stats(self.h, C.byref(s)), "get_stats")
         return s
 
+    def schedule(self):
+        """(acti_time, acti_counter) per LTS level (integrator.hpp:48-49)."""
+        t = (C.c_double * MAX_LEVEL)()
+        c = (C.c_int32 * MAX_LEVEL)()
+        self._check(self.lib.dacti_get_schedule(self.h, t, c), "get_schedule")
+        return list(t), list(c)
+
     def set_launch(self, block=256, grid=0, graphs=False):
         """Launch configuration of this integrator's stepping (results do not depend on it)."""
         self._check(self.lib.dacti_set_launch(self.h, block, grid, int(bool(graphs))), "set_launch")

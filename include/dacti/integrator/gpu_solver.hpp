@@ -1,18 +1,16 @@
-// dacti/integrator.hpp -- C++ facade over the C ABI (dacti_gpu.h).
+// dacti/integrator/gpu_solver.hpp -- untyped C++ owner of one GPU handle.
 //
-// Mirrors the public surface of the reference's
+// GpuSolver wraps the C ABI (dacti_gpu.h): construction runs the
+// integrator's initialisation (integrator/integrator.hpp:72-163 of the
+// reference) on the GPU, integrate() is one global step
+// (impl/integrate.hpp:11-87), get_nodes() copies the node pool to the host.
+// Model, limiter and estimator weights come from the dacti_case fields.
 // dacti::integrator::Integrator<Model, Limiter, ErrorEstimator>
-// (src/lib/integrator/integrator.hpp:14-225) for a C++ driver: construction
-// runs the constructor's initialisation (integrator.hpp:72-163) on the GPU,
-// integrate() is one global step (impl/integrate.hpp:11-87), and
-// n_active_cells / get_nodes / for_each_active_cell / global_dt /
-// max_acti_level keep their names and meaning.  Model and limiter are chosen
-// by the case (dacti_case::model, ::limiter) instead of template arguments;
-// the error estimator is the weighted Laplacian with dacti_case::weights
-// (integrator/error_estimators.hpp:10-47).  Errors throw dacti::Error; a NaN
-// state is reported through force_break as in impl/cell_update.hpp:52-57.
-#ifndef DACTI_INTEGRATOR_HPP
-#define DACTI_INTEGRATOR_HPP
+// (integrator/integrator.hpp in this directory) is the reference-shaped
+// interface on top of it.  Errors throw dacti::Error; a NaN state is
+// reported through force_break as in impl/cell_update.hpp:52-57.
+#ifndef DACTI_GPU_SOLVER_HPP
+#define DACTI_GPU_SOLVER_HPP
 
 #include <cstdint>
 #include <stdexcept>
@@ -20,7 +18,7 @@
 #include <utility>
 #include <vector>
 
-#include "../dacti_gpu.h"
+#include "../../dacti_gpu.h"
 
 namespace dacti {
 
@@ -56,19 +54,19 @@ struct NodePool {
     const double *cons(int64_t i) const { return &u[i * sz.num_eqns]; }
 };
 
-class Integrator {
+class GpuSolver {
 public:
     static constexpr int NumMeta = 4;  // acti_level, error, max_dt, cfl (integrator.hpp:30)
     static constexpr int MAX_LEVEL = DACTI_MAX_LEVEL;
 
-    Integrator(const dacti_case &c, const dacti_tree &t) : sz_(model_sizes(c.model, c.dim)) {
+    GpuSolver(const dacti_case &c, const dacti_tree &t) : sz_(model_sizes(c.model, c.dim)) {
         int rc = dacti_create(&c, &t, &h_);
         if (rc != DACTI_OK) throw Error(rc, dacti_last_error(nullptr));
     }
-    Integrator(const Integrator &) = delete;
-    Integrator &operator=(const Integrator &) = delete;
-    Integrator(Integrator &&o) noexcept : h_(std::exchange(o.h_, nullptr)), sz_(o.sz_) {}
-    ~Integrator() {
+    GpuSolver(const GpuSolver &) = delete;
+    GpuSolver &operator=(const GpuSolver &) = delete;
+    GpuSolver(GpuSolver &&o) noexcept : h_(std::exchange(o.h_, nullptr)), sz_(o.sz_) {}
+    ~GpuSolver() {
         if (h_) dacti_destroy(h_);
     }
 
@@ -135,6 +133,22 @@ public:
         dacti_stats s{};
         check(dacti_get_stats(h_, &s));
         return s;
+    }
+    // LTS schedule state: acti_time[L], acti_counter[L] (integrator.hpp:48-49)
+    void schedule(double *acti_time, int32_t *acti_counter) const {
+        check(dacti_get_schedule(h_, acti_time, acti_counter));
+    }
+    // one global step with fn(user, handle, level, acti_time, n_substeps, dt)
+    // called after every substep (impl/integrate.hpp:64)
+    double integrate_hooked(dacti_substep_hook fn, void *user, bool &force_break, bool signal = true) {
+        check(dacti_set_substep_hook(h_, fn, user));
+        double dt = 0.0;
+        int32_t fb = 0;
+        const int rc = dacti_integrate(h_, signal ? 1 : 0, &dt, &fb);
+        dacti_set_substep_hook(h_, nullptr, nullptr);
+        check(rc);
+        force_break = fb != 0;
+        return dt;
     }
     double global_dt() const { return stats().global_dt; }
     int max_acti_level() const { return stats().max_acti_level; }

@@ -185,6 +185,11 @@ int dacti_export_nodes(dacti_handle *h, const dacti_node_view *v);
 
 int dacti_get_stats(dacti_handle *h, dacti_stats *s);
 
+/* LTS schedule state, the public acti_time / acti_counter arrays of
+ * Integrator (integrator/integrator.hpp:48-49): DACTI_MAX_LEVEL entries
+ * each; either pointer may be NULL */
+int dacti_get_schedule(dacti_handle *h, double *acti_time, int32_t *acti_counter);
+
 /* turn HIP-event timing of the per-substep kernels on or off */
 int dacti_set_timing(dacti_handle *h, int32_t on);
 

@@ -158,9 +158,48 @@ def test_loading_requires_built_library(monkeypatch, tmp_path):
 
 
 CPP_PROBE = r"""
-#include "dacti/integrator.hpp"
+#include "dacti/integrator/integrator.hpp"
 #include <cstdio>
-int main() {
+using namespace dacti;
+static const char *CASE = R"(
+[scene]
+name = "probe"
+domain_size = [1.0, 1.0]
+[run]
+print_interval = 1.0
+end_time = 1.0
+[integrator]
+min_refinement_level = 3
+max_refinement_level = 4
+error_threshold = [0.01, 0.1]
+max_cfl = 0.5
+[model.reference_values]
+rho = 1.0
+p = 1.0
+vel = [0.0, 0.0]
+[initial_condition]
+type = "uniform"
+[boundary_conditions.X]
+type = "periodic"
+[boundary_conditions.Y]
+type = "periodic"
+)";
+int probe_integrator(const char *out_root) {
+    // the reference's construction by class-template argument deduction;
+    // without a GPU the handle cannot be created and this throws
+    auto config = config::Config<model::euler<2>>::from_string(CASE, out_root);
+    scene::Scene<model::euler<2>> scene(config);
+    try {
+        integrator::Integrator integ{integrator::slope::minmod{},
+                                     integrator::error::weighted_laplacian{config->error_weights}, scene, config};
+        std::printf("created %zu\n", integ.n_active_cells());
+    } catch (const dacti::Error &e) {
+        std::printf("create failed %d\n", e.code);
+    }
+    return (int) scene.init_tree().n;
+}
+int main(int, char **argv) {
+    std::printf("tree %d\n", probe_integrator(argv[1]));
     auto s = dacti::model_sizes(DACTI_MODEL_MULTI, 3);
     std::printf("%d %d %d %d %d\n", s.dim, s.num_eqns, s.num_flux, s.num_parm, s.num_grad);
     try {
@@ -184,6 +223,8 @@ def test_cpp_facade_compiles_and_links(tmp_path):
     libdir = os.path.dirname(_lib.LIB_PATH)
     subprocess.run(["g++", "-std=c++17", "-Wall", "-Wextra", "-Werror", "-I", os.path.join(ROOT, "include"),
                     str(src), "-L", libdir, "-ldacti_gpu", f"-Wl,-rpath,{libdir}", "-o", str(exe)], check=True)
-    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split("\n")
-    assert out[0] == "3 7 11 7 6"
-    assert out[1] == "error -1"
+    out = subprocess.run([str(exe), str(tmp_path)], check=True, capture_output=True, text=True,
+                         timeout=120).stdout.split("\n")
+    assert out[0] == "create failed -2" and out[1] == f"tree {2 + 4 + 16 + 64}"  # null node, root, levels 1-3
+    assert out[2] == "3 7 11 7 6"
+    assert out[3] == "error -1"

@@ -1,8 +1,8 @@
-"""The C++ facade (include/dacti/integrator.hpp) driving a case on the GPU.
+"""The untyped C++ owner of a GPU handle (include/dacti/integrator/gpu_solver.hpp) driving a case on the GPU.
 
 A small C++ driver, compiled here against the header and libdacti_gpu.so,
 reads a case and its initial tree written by Python, runs two global steps
-through dacti::Integrator and writes the node pool's conserved state; it must
+through dacti::GpuSolver and writes the node pool's conserved state; it must
 equal the oracle's bit for bit (the same path as the Python facade)."""
 import ctypes as C
 import os
@@ -19,7 +19,7 @@ pytestmark = pytest.mark.gpu
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 DRIVER = r"""
-#include "dacti/integrator.hpp"
+#include "dacti/integrator/gpu_solver.hpp"
 #include <cstdio>
 #include <vector>
 template <typename T> std::vector<T> rd(FILE *f, size_t n) { std::vector<T> v(n); if (fread(v.data(), sizeof(T), n, f) != n) throw 1; return v; }
@@ -35,7 +35,7 @@ int main(int argc, char **argv) {
     auto ctr = rd<double>(f, n * c.dim), size = rd<double>(f, n), p = rd<double>(f, n * sz.num_eqns);
     std::fclose(f);
     dacti_tree t{n, parent.data(), cb.data(), lev.data(), fl.data(), bc.data(), ctr.data(), size.data(), p.data()};
-    dacti::Integrator g(c, t);
+    dacti::GpuSolver g(c, t);
     bool brk = false;
     for (int s = 0; s < 2; ++s) g.integrate(brk);
     dacti::NodePool P = g.get_nodes();
