@@ -43,6 +43,7 @@ def bubble_text(end_time, print_interval):
         base=4, top=6, emin="0.02", emax="0.05", rho2="0.138", shock_x="-0.7", r2=repr(case.bcs[0].value[0]),
         u2=repr(case.bcs[0].value[2]), p2=repr(case.bcs[0].value[4]), radius="0.25", bx="-0.3")
     text = text.replace("alpha = 1.0\n", "alpha = 1.0\nWe = 20.0\n", 1)
+    text = text.replace("max_cfl = 0.5\n", "max_cfl = 0.5\nmax_dt = 1.0\n", 1)  # dt independent of the frames
     text = text.replace("print_interval = 1.0", f"print_interval = {print_interval!r}")
     return text.replace("end_time = 1.0", f"end_time = {end_time!r}")
 
@@ -71,7 +72,7 @@ def test_default_driver_frames_and_observables(run_case_exe, tmp_path):
     dt, _ = o.integrate()
     del o
     steps = 3
-    text = bubble_text(dt * (steps - 0.5), dt * 0.99)  # a frame after every step, three steps
+    text = bubble_text(dt * (steps - 0.5), dt * 0.5)  # a frame after every step, three steps
     (tmp_path / "cases").mkdir()
     path = tmp_path / "cases" / "bubble.toml"
     path.write_text(text)
