@@ -40,12 +40,19 @@ class CaseStruct(C.Structure):
     ]
 
 
+# object boundary record (dacti_boundary): type, unit wall normal, inflow state, fade-in
+RECORD_DTYPE = np.dtype([("type", np.int32), ("pad", np.int32), ("normal", np.float64, 3), ("p", np.float64, 8),
+                         ("fade_in", np.float64)])
+BC_OBJECT = 7  # bc code of a face whose condition is its node's record
+
+
 class TreeStruct(C.Structure):
     _fields_ = [
         ("n_nodes", C.c_int64),
         ("parent", C.POINTER(C.c_int32)), ("child_base", C.POINTER(C.c_int32)),
         ("tree_level", C.POINTER(C.c_int8)), ("flags", C.POINTER(C.c_uint8)), ("bc", C.POINTER(C.c_uint8)),
         ("center", C.POINTER(C.c_double)), ("size", C.POINTER(C.c_double)), ("p", C.POINTER(C.c_double)),
+        ("brec", C.POINTER(C.c_int32)), ("n_records", C.c_int64), ("records", C.c_void_p),
     ]
 
 
@@ -86,6 +93,11 @@ SIGNATURES = {
     "dacti_set_remesh": (C.c_int, [C.c_void_p, C.c_int32]),
     "dacti_mark_split": (C.c_int, [C.c_void_p, C.POINTER(C.c_int32), C.c_int64]),
     "dacti_destroy": (C.c_int, [C.c_void_p]),
+    "dacti_scene_build": (C.c_int, [C.c_char_p, C.c_char_p, C.c_int32, C.c_int32, C.c_int32, C.c_void_p, C.c_void_p,
+                                    C.POINTER(C.c_void_p)]),
+    "dacti_scene_case": (C.c_int, [C.c_void_p, C.POINTER(CaseStruct)]),
+    "dacti_scene_tree": (C.c_int, [C.c_void_p, C.POINTER(TreeStruct)]),
+    "dacti_scene_destroy": (C.c_int, [C.c_void_p]),
     "dacti_last_error": (C.c_char_p, [C.c_void_p]),
     "dacti_batch_create": (C.c_int, [C.POINTER(C.c_void_p), C.c_int32, C.POINTER(C.c_void_p)]),
     "dacti_batch_integrate": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_int32)]),
@@ -181,6 +193,10 @@ def tree_struct(tree):
     t.center = ptr(tree.center, C.c_double)
     t.size = ptr(tree.size, C.c_double)
     t.p = ptr(tree.p, C.c_double)
+    if getattr(tree, "brec", None) is not None and len(tree.records):
+        t.brec = ptr(tree.brec, C.c_int32)
+        t.n_records = len(tree.records)
+        t.records = tree.records.ctypes.data
     return t
 
 

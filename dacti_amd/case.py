@@ -8,6 +8,7 @@ model/euler.hpp:288-324, model/navier_stokes.hpp:120-134 and
 model/multi.hpp:474-549.  Quirks of the reference are kept and marked.
 """
 import math
+import os
 from dataclasses import dataclass, field
 from typing import List, Optional, Tuple
 
@@ -38,12 +39,15 @@ class BC:
 
 @dataclass
 class Geometry:
-    """Built-in analytic shape: circle (2D) or sphere (3D)."""
+    """A [[geometry]] entry (config/config.cpp:66-125): a built-in shape or
+    an STL file ("mesh"), its type ("solid", "droplet", "bubble", ...) and
+    translation.  The scene is built from the case's TOML text by the C++
+    builder, which reads every key of the entry."""
     name: str
-    type: str          # "bubble" or "droplet" (fluid objects without a boundary condition)
+    type: str
     shape: str
     center: Tuple[float, float, float]
-    radius: float
+    radius: float = 0.0
 
 
 @dataclass
@@ -77,6 +81,8 @@ class Case:
     ic_neg: Tuple[float, ...] = ()
     ic_pos: Tuple[float, ...] = ()
     ic_grid: object = None  # (n,..., NE) primitive array for ic_type == "grid"
+    toml_text: str = None   # the case file's text (the scene builder reads the objects from it)
+    base_dir: str = "."     # directory geometry paths are relative to
     geometry: List[Geometry] = field(default_factory=list)
     zones: List[Tuple[Tuple[float, float, float], Tuple[float, float, float], int]] = field(default_factory=list)
 
@@ -323,15 +329,22 @@ def load_case_table(tbl, model, dim, limiter="minmod"):
             bcs[2 * d + 1] = _parse_bc(case, sub.get("pos", {}), ne)
     case.periodic = tuple(periodic)
     case.bcs = bcs
-    # geometry: config/config.cpp:66-125 (built-in analytic shapes of fluid objects)
+    # geometry: config/config.cpp:66-125 (shapes, meshes and their boundary
+    # conditions are read by the C++ scene builder from the same text)
     for g in tbl.get("geometry", []) or []:
+        name = get_entry(g, "name", "str")
         gtype = get_entry(g, "type", "str", default="solid")
-        if gtype == "solid" or "path" in g:
-            raise ConfigError("solid / mesh geometry is not supported by this engine")
         tr = get_entry(g, "translation", "vec", 3, default=(0.0, 0.0, 0.0))
-        shape = get_entry(g, "shape", "str")
-        radius = get_entry(g, "radius", "float")
-        case.geometry.append(Geometry(g.get("name", ""), gtype, shape, tr, radius))
+        if "path" in g:
+            shape = "mesh"
+        elif "shape" in g:
+            shape = get_entry(g, "shape", "str")
+        else:
+            raise ConfigError(f"Geometry '{name}' needs to either have an associated file path or a shape.")
+        if gtype == "solid" and not (isinstance(bct, dict) and name in bct):
+            raise ConfigError(f"Must define boundary condition for solid geometry '{name}'")
+        radius = g.get("radius", 0.0)
+        case.geometry.append(Geometry(name, gtype, shape, tuple(tr), float(radius) if radius else 0.0))
     for z in tbl.get("refinement_zone", []) or []:
         case.zones.append((get_entry(z, "start", "vec", 3), get_entry(z, "end", "vec", 3),
                            get_entry(z, "max_refinement_level", "uint")))
@@ -339,10 +352,14 @@ def load_case_table(tbl, model, dim, limiter="minmod"):
 
 
 def load_case(path, model, dim, limiter="minmod"):
-    with open(path, "rb") as f:
-        tbl = _toml.load(f)
-    return load_case_table(tbl, model, dim, limiter)
+    with open(path, "r") as f:
+        text = f.read()
+    case = load_case_table(_toml.loads(text), model, dim, limiter)
+    case.toml_text, case.base_dir = text, os.path.dirname(os.path.abspath(path))
+    return case
 
 
-def loads_case(text, model, dim, limiter="minmod"):
-    return load_case_table(_toml.loads(text), model, dim, limiter)
+def loads_case(text, model, dim, limiter="minmod", base_dir="."):
+    case = load_case_table(_toml.loads(text), model, dim, limiter)
+    case.toml_text, case.base_dir = text, base_dir
+    return case

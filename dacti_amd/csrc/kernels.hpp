@@ -20,15 +20,6 @@ enum : uint8_t {
 };
 enum : uint8_t { MK_SPLIT = 1, MK_UNSPLIT_CAND = 2, MK_UNSPLIT_CHILD = 4, MK_READY = 8, MK_DEFER = 16, MK_RA_DONE = 32 };
 
-// Pool arrays are global-memory pointers (address space 1): loads through
-// them are global_load instructions, not flat ones (a flat load also counts
-// against the scalar-load counter, so every wait for a scalar load would
-// drain the vector loads in flight).
-#if defined(__HIP_DEVICE_COMPILE__)
-#define GMEM __attribute__((address_space(1)))
-#else
-#define GMEM
-#endif
 struct Dev {
     int64_t C;
     GMEM int32_t *parent, *child_base, *nbr;  // nbr[f*C + i]
@@ -46,6 +37,8 @@ struct Dev {
     GMEM double *error, *ctime, *max_dt, *cfl;
     GMEM double *scratch;                     // C doubles (smoothing / reductions)
     GMEM int32_t *err;                        // device error word (subtree deeper than supported)
+    GMEM int32_t *brec;                       // object boundary record of a node's BC_OBJECT faces
+    const GMEM BndRec *brecs;                 // the records (fixed at construction)
 };
 
 // device status block read back by the host after a substep phase
@@ -463,18 +456,30 @@ __device__ __noinline__ void owner_face_flux_with(const Dev &d, const MConst &c,
                               dir, dt, F);
 }
 
-// boundary_flux: integrator/boundary.hpp:130-160 (already scaled)
+// ghost state of face f of cell y: the domain condition the face carries,
+// or the cell's object record (integrator/boundary.hpp:75-123)
 template <int MODEL, int D>
-__device__ inline void boundary_flux(const Dev &d, const MConst &c, int32_t y, int face, int dim, int dir,
-                                     double dt, double *out) {
+__device__ inline void boundary_value_at(const Dev &d, const MConst &c, int32_t y, int f, const double *pc, double t,
+                                         double *pb) {
+    const uint8_t b = d.bc[f * d.C + y];
+    if (b == BC_OBJECT) boundary_value_rec<MODEL, D>(c, &d.brecs[d.brec[y]], pc, t, pb);
+    else boundary_value<MODEL, D>(c, b ? b - 1 : f, pc, t, pb);
+}
+
+// boundary_flux: integrator/boundary.hpp:130-160 (already scaled), face f
+// of cell y
+template <int MODEL, int D>
+__device__ inline void boundary_flux(const Dev &d, const MConst &c, int32_t y, int f, int dim, int dir, double dt,
+                                     double *out) {
     using S = Sz<MODEL, D>;
     constexpr int NE = S::NE, NF = S::NF, IU = S::IU, IP = S::IP;
     const int64_t C = d.C;
     double pc[NE], pb[NE], F[NF];
     for (int e = 0; e < NE; ++e) pc[e] = d.p[e * C + y];
     const double t0 = d.t[y], t1 = t0 + dt;
-    boundary_value<MODEL, D>(c, face, pc, t0, pb);
-    const int type = c.bc_type[face];
+    boundary_value_at<MODEL, D>(d, c, y, f, pc, t0, pb);
+    const uint8_t b = d.bc[f * C + y];
+    const int type = b == BC_OBJECT ? d.brecs[d.brec[y]].type : c.bc_type[b ? b - 1 : f];
     if (type == BC_FREESLIP) {
         flux_F<MODEL, D, true>(c, pc, pb[IU + dim], dim, F);
     } else if (type == BC_NOSLIP) {
@@ -496,11 +501,6 @@ __device__ inline void boundary_flux(const Dev &d, const MConst &c, int32_t y, i
     const double direction = 2 * dir - 1;
     const double area = c.area[d.level[y]];
     for (int e = 0; e < NF; ++e) out[e] = F[e] * area * (t1 - t0) * direction;
-}
-
-__device__ __forceinline__ int bc_face(const Dev &d, int32_t y, int f) {
-    const uint8_t b = d.bc[f * d.C + y];
-    return b ? b - 1 : f;
 }
 
 // ------------------------------------------------------------------ flux
@@ -546,7 +546,7 @@ __device__ void flux_cell(const Dev &d, const MConst &c, int32_t y, int L, doubl
 #pragma unroll
             for (int e = 0; e < NF; ++e) cc[e] = 0.0;
             if (nb == 0) {
-                boundary_flux<MODEL, D>(d, c, y, bc_face(d, y, f), dim, dir, dt, F);
+                boundary_flux<MODEL, D>(d, c, y, f, dim, dir, dt, F);
 #pragma unroll
                 for (int e = 0; e < NF; ++e) cc[e] = cc[e] - F[e];
             } else if (d.flags[nb] & F_LEAF) {
@@ -729,7 +729,7 @@ __device__ __forceinline__ void flux_simple_body(const Dev &d, const MConst &c, 
 #pragma unroll
             for (int e = 0; e < NF; ++e) cc[e] = 0.0;
             if (nb == 0) {
-                boundary_flux<MODEL, D>(d, c, y, bc_face(d, y, f), dim, dir, dt, F);
+                boundary_flux<MODEL, D>(d, c, y, f, dim, dir, dt, F);
 #pragma unroll
                 for (int e = 0; e < NF; ++e) cc[e] = cc[e] - F[e];
             } else {
@@ -865,7 +865,7 @@ __device__ inline void compute_slope_dev(const Dev &d, const MConst &c, int32_t 
         double pn[NE], pp[NE], xn, xp, tn, tp;
         const double xc = 0.0;
         if (in == 0) {
-            boundary_value<MODEL, D>(c, bc_face(d, y, 2 * dim), pc, yt, pn);
+            boundary_value_at<MODEL, D>(d, c, y, 2 * dim, pc, yt, pn);
             xn = -ys;
             tn = yt;
         } else {
@@ -884,7 +884,7 @@ __device__ inline void compute_slope_dev(const Dev &d, const MConst &c, int32_t 
             xn = -0.5 * (ys + d.size[in]);
         }
         if (ip == 0) {
-            boundary_value<MODEL, D>(c, bc_face(d, y, 2 * dim + 1), pc, yt, pp);
+            boundary_value_at<MODEL, D>(d, c, y, 2 * dim + 1, pc, yt, pp);
             xp = ys;
             tp = yt;
         } else {
@@ -988,7 +988,7 @@ __device__ __forceinline__ void sweep_side(const Dev &d, const MConst &c, int32_
     const int32_t nb = d.nbr[(2 * dim + DIR) * C + y];
     hv = true;
     if (nb == 0) {
-        boundary_value<MODEL, D>(c, bc_face(d, y, 2 * dim + DIR), pc, yt, pv);
+        boundary_value_at<MODEL, D>(d, c, y, 2 * dim + DIR, pc, yt, pv);
         xv = DIR ? ys : -ys;
         tv = yt;
         hv = false;
@@ -1148,7 +1148,7 @@ __device__ inline double bc_alpha(const Dev &d, const MConst &c, int32_t y, int 
     using S = Sz<MODEL, D>;
     double pc[S::NE], pb[S::NE];
     for (int e = 0; e < S::NE; ++e) pc[e] = d.p[e * d.C + y];
-    boundary_value<MODEL, D>(c, bc_face(d, y, f), pc, d.t[y], pb);
+    boundary_value_at<MODEL, D>(d, c, y, f, pc, d.t[y], pb);
     return pb[S::IA];
 }
 
@@ -1377,6 +1377,7 @@ __device__ __forceinline__ void split_body(const Dev &d, const MConst &c, const 
             d.bc[f * C + x] = (((ch >> a) & 1) == dir) ? d.bc[f * C + y] : 0;
             for (int e = 0; e < S::NF; ++e) d.acc[(f * S::NF + e) * C + x] = 0.0;
         }
+        d.brec[x] = d.brec[y];
         d.level[x] = (int8_t) (plev + 1);
         d.acti[x] = (int8_t) L;
         d.cluster[x] = (int8_t) L;

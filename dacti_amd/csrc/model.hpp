@@ -10,6 +10,16 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+// Pool arrays are global-memory pointers (address space 1): loads through
+// them are global_load instructions, not flat ones (a flat load also counts
+// against the scalar-load counter, so every wait for a scalar load would
+// drain the vector loads in flight).
+#if defined(__HIP_DEVICE_COMPILE__)
+#define GMEM __attribute__((address_space(1)))
+#else
+#define GMEM
+#endif
+
 namespace dacti {
 
 enum { EULER = 0, NAVIER_STOKES = 1, MULTI = 2 };
@@ -51,6 +61,8 @@ struct MConst {
 };
 
 enum { BC_ZERO_GRADIENT = 0, BC_FREESLIP, BC_NOSLIP, BC_INFLOW, BC_INFLOW_R, BC_OUTFLOW, BC_DIRICHLET };
+// bc code of a face whose condition is its node's object record
+enum : uint8_t { BC_OBJECT = 7 };
 
 __device__ __forceinline__ double smin(double a, double b) { return (b < a) ? b : a; }  // std::min
 __device__ __forceinline__ double smax(double a, double b) { return (a < b) ? b : a; }  // std::max
@@ -421,6 +433,14 @@ __device__ inline double limiter(int id, double s1, double s2) {
     }
 }
 
+// an object boundary condition: the C ABI's dacti_boundary
+struct BndRec {
+    int32_t type, pad;
+    double normal[3];
+    double p[8];
+    double fade_in;
+};
+
 // ghost state of a domain boundary face: integrator/boundary.hpp:75-123
 template <int MODEL, int D>
 __device__ inline void boundary_value(const MConst &c, int face, const double *pc, double t, double *pb) {
@@ -452,6 +472,42 @@ __device__ inline void boundary_value(const MConst &c, int face, const double *p
     }
     if (type == BC_DIRICHLET) {
         for (int i = 0; i < S::NE; ++i) pb[i] = c.bc_p[face][i];
+        return;
+    }
+    for (int i = 0; i < S::NE; ++i) pb[i] = pc[i];
+}
+
+// ghost state of an object boundary (record r): integrator/boundary.hpp:75-123
+// with the record's wall normal and inflow state
+template <int MODEL, int D>
+__device__ inline void boundary_value_rec(const MConst &c, const GMEM BndRec *r, const double *pc, double t,
+                                          double *pb) {
+    using S = Sz<MODEL, D>;
+    const int type = r->type;
+    if (type == BC_FREESLIP) {
+        double dot = 0;
+        for (int i = 0; i < S::NE; ++i) pb[i] = pc[i];
+        for (int i = 0; i < D; ++i) dot += pc[S::IU + i] * r->normal[i];
+        for (int i = 0; i < D; ++i) pb[S::IU + i] = pc[S::IU + i] - r->normal[i] * dot;
+        return;
+    }
+    if (type == BC_NOSLIP) {
+        for (int i = 0; i < S::NE; ++i) pb[i] = pc[i];
+        for (int i = 0; i < D; ++i) pb[S::IU + i] *= -1.0;
+        return;
+    }
+    if (type == BC_INFLOW || type == BC_INFLOW_R) {
+        for (int i = 0; i < S::NE; ++i) pb[i] = r->p[i];
+        if (!supersonic<MODEL, D>(c, pc)) pb[S::IP] = pc[S::IP];
+        const double fi = r->fade_in;
+        if (fi > 0.0) {
+            const double vm = smin(t / fi, 1.0);
+            for (int i = 0; i < D; ++i) pb[S::IU + i] *= vm;
+        }
+        return;
+    }
+    if (type == BC_DIRICHLET) {
+        for (int i = 0; i < S::NE; ++i) pb[i] = r->p[i];
         return;
     }
     for (int i = 0; i < S::NE; ++i) pb[i] = pc[i];

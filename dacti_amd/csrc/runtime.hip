@@ -30,7 +30,7 @@ std::vector<Field> fields_of(dacti_handle *h, Dev &d) {
         {(void **) &d.dep, 8, s.D * s.NF},
         {(void **) &d.error, 8, 1},          {(void **) &d.ctime, 8, 1},
         {(void **) &d.max_dt, 8, 1},         {(void **) &d.cfl, 8, 1},
-        {(void **) &d.scratch, 8, 1},
+        {(void **) &d.scratch, 8, 1},        {(void **) &d.brec, 4, 1},
     };
 }
 
@@ -99,6 +99,7 @@ int alloc_pool(dacti_handle *h, int64_t C, Dev *old, int64_t n) {
     h->tmp_bytes = std::max(b1, std::max(b2, b3)) + 1024;
     HIPCHK(h, hipMalloc(&h->d_tmp, h->tmp_bytes));
     nd.err = (GMEM int32_t *) &h->d_status_home->dev_error;
+    nd.brecs = (const GMEM BndRec *) h->d_brecs;
     h->d = nd;
     return DACTI_OK;
 }
@@ -200,6 +201,7 @@ int gc(dacti_handle *h) {
     HIPCHK(h, hipMemsetAsync(nd.buffer + n_new, 0xff, (size_t) (nd.C - n_new), h->stream));
     HIPCHK(h, hipMemsetAsync(nd.acti + n_new, 0xff, (size_t) (nd.C - n_new), h->stream));
     nd.err = (GMEM int32_t *) &h->d_status_home->dev_error;
+    nd.brecs = (const GMEM BndRec *) h->d_brecs;
     k_gc_remap<<<blocks(n_new), 256, 0, h->stream>>>(nd, n_new, h->sz.NFACE, keep, o2n);
     HIPCHK(h, hipGetLastError());
     HIPCHK(h, hipStreamSynchronize(h->stream));
@@ -344,7 +346,7 @@ void destroy_handle(dacti_handle *h) {
     batch_free(h->solo);
     free_dev(h, h->d);
     for (void *p : {(void *) h->d_list, (void *) h->d_list2, (void *) h->d_off, (void *) h->d_rn, (void *) h->d_nsel, h->d_tmp,
-                    (void *) h->d_hist, (void *) h->d_status})
+                    (void *) h->d_hist, (void *) h->d_status, (void *) h->d_brecs})
         if (p) (void) hipFree(p);
     if (h->h_status) (void) hipHostFree(h->h_status);
     if (h->h_nsel) (void) hipHostFree(h->h_nsel);
@@ -415,6 +417,25 @@ int dacti_create(const dacti_case *c, const dacti_tree *t, dacti_handle **out) {
     if (c->n_zones < 0 || c->n_zones > DACTI_MAX_ZONES || c->max_cfl <= 0) {
         g_create_error = "dacti_create: invalid case parameters";
         return DACTI_E_ARG;
+    }
+    {
+        // bc codes: 0, 1 + a domain face, or DACTI_BC_OBJECT with a valid record
+        const int nface = 2 * c->dim;
+        if (t->n_records < 0 || (t->n_records > 0 && !t->records)) {
+            g_create_error = "dacti_create: invalid boundary records";
+            return DACTI_E_ARG;
+        }
+        for (int64_t i = 0; i < t->n_nodes; ++i)
+            for (int f = 0; f < nface; ++f) {
+                const uint8_t b = t->bc[(size_t) i * nface + f];
+                const bool ok = b <= nface || (b == DACTI_BC_OBJECT && t->brec && t->brec[i] >= 0 &&
+                                               t->brec[i] < t->n_records);
+                if (!ok) {
+                    g_create_error = "dacti_create: node " + std::to_string(i) + " face " + std::to_string(f) +
+                                     " has bc code " + std::to_string(b) + " without a valid record";
+                    return DACTI_E_ARG;
+                }
+            }
     }
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
@@ -506,6 +527,14 @@ int dacti_create(const dacti_case *c, const dacti_tree *t, dacti_handle **out) {
         return fail(DACTI_E_MEMORY);
     }
     h->d_status_home = h->d_status;
+    if (t->n_records > 0) {
+        if (hipMalloc(&h->d_brecs, sizeof(BndRec) * (size_t) t->n_records) != hipSuccess ||
+            hipMemcpy(h->d_brecs, t->records, sizeof(BndRec) * (size_t) t->n_records, hipMemcpyHostToDevice) !=
+                hipSuccess) {
+            h->err = "boundary record upload failed";
+            return fail(DACTI_E_MEMORY);
+        }
+    }
     const int64_t n = t->n_nodes;
     h->n = n;
     const int64_t C = n + std::max<int64_t>(n / 2, 65536);
@@ -521,6 +550,11 @@ int dacti_create(const dacti_case *c, const dacti_tree *t, dacti_handle **out) {
         (rc = upload<double>(h, d.center, t->center, n, s.D, C)) ||
         (rc = upload<double>(h, d.size, t->size, n, 1, C)) || (rc = upload<double>(h, d.p, t->p, n, s.NE, C)))
         return fail(rc);
+    {
+        std::vector<int32_t> br((size_t) n, -1);
+        if (t->brec) std::copy(t->brec, t->brec + n, br.begin());
+        if ((rc = upload<int32_t>(h, d.brec, br.data(), n, 1, C))) return fail(rc);
+    }
     if (hipMemset(d.cluster, 0xff, (size_t) C) != hipSuccess || hipMemset(d.buffer, 0xff, (size_t) C) != hipSuccess ||
         hipMemset(d.acti, 0xff, (size_t) C) != hipSuccess) {
         h->err = "memset failed";

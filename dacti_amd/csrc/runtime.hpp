@@ -21,6 +21,10 @@
 #include "../../include/dacti_gpu.h"
 #include "kernels.hpp"
 
+static_assert(sizeof(dacti::BndRec) == sizeof(dacti_boundary) && offsetof(dacti::BndRec, p) == offsetof(dacti_boundary, p) &&
+                  offsetof(dacti::BndRec, fade_in) == offsetof(dacti_boundary, fade_in),
+              "BndRec mirrors dacti_boundary");
+
 using namespace dacti;
 
 namespace dacti_rt {
@@ -77,6 +81,7 @@ struct dacti_handle {
     int32_t *d_off = nullptr;
     uint8_t *d_rn = nullptr;
     int32_t *d_nsel = nullptr;
+    BndRec *d_brecs = nullptr;  // object boundary records (fixed at construction)
     void *d_tmp = nullptr;
     size_t tmp_bytes = 0;
     unsigned long long *d_hist = nullptr;

@@ -1,19 +1,23 @@
 """Initial tree of a case: Scene<M>::init_domain (src/lib/scene/scene.cpp:369-431).
 
-The tree is grown level by level as the reference does (compute signed
-distances, boundaries, split detection, split), vectorised over each
-level's new nodes.  Children of one split occupy consecutive indices and
-split parents are taken in node order.  Scope: domain boundary conditions,
-periodic axes, uniform / jump / grid initial conditions, analytic
-fluid objects (circle, sphere) of type "bubble" or "droplet", and
-axis-aligned refinement zones.  Solid geometry (libigl signed distance,
-embree ray casts, scene.cpp:47-147) is outside this engine's scope.
+Cases with objects ([[geometry]]: built-in shapes, STL meshes, fluid
+objects and solids with their boundary conditions) are built by the C++
+implementation behind the C ABI (dacti_scene_build, include/dacti/config),
+from the case's TOML text.  Cases without objects are grown here, level by
+level as the reference does (boundaries, split detection, split),
+vectorised over each level's new nodes; the C++ builder gives the same
+tree bit for bit.  Children of one split occupy consecutive indices and
+split parents are taken in node order.  Scope here: domain boundary
+conditions, periodic axes, uniform / jump / grid initial conditions and
+axis-aligned refinement zones.
 """
+import ctypes as C
 from dataclasses import dataclass
 
 import numpy as np
 
-from ._lib import (F_ACTIVE, F_BOUNDARY, F_CAN_SPLIT, F_CAN_UNSPLIT, F_LEAF, F_SHOULD_SPLIT, MODEL_MULTI)
+from . import _lib
+from ._lib import F_ACTIVE, F_CAN_SPLIT, F_CAN_UNSPLIT, F_LEAF, RECORD_DTYPE
 from .case import BC_TYPES
 
 
@@ -28,6 +32,8 @@ class Tree:
     center: np.ndarray      # f64   [n, dim]
     size: np.ndarray        # f64   [n]
     p: np.ndarray           # f64   [n, num_eqns]
+    brec: np.ndarray = None     # int32 [n]: object boundary record of a node's object faces (-1 none)
+    records: np.ndarray = None  # RECORD_DTYPE [n_records]
 
     def leaves(self):
         f = self.flags
@@ -43,21 +49,9 @@ def _quadrant(D):
     return q
 
 
-def _signed_distance(case, center):
-    """Nearest analytic object: (distance, object index); +max without objects."""
-    n = center.shape[0]
-    sd = np.full(n, np.finfo(np.float64).max)
-    near = np.zeros(n, np.int64)
-    for mi, g in enumerate(case.geometry):
-        c = np.asarray(g.center[:case.dim], dtype=np.float64)
-        dist = np.sqrt(((center - c) ** 2).sum(axis=1)) - g.radius
-        better = dist < sd
-        sd = np.where(better, dist, sd)
-        near = np.where(better, mi, near)
-    return sd, near
-
-
 def build_tree(case):
+    if case.geometry:
+        return _build_tree_cpp(case)
     D = case.dim
     NE = case.num_eqns
     NCH = 1 << D
@@ -70,8 +64,6 @@ def build_tree(case):
     size = [np.array([0.0, case.side_length])]
     leaf = [np.array([False, True])]
     can_unsplit = [np.array([False, False])]
-    sdist = [np.array([-1.0, 0.0])]
-    nearest = [np.zeros(2, np.int64)]
     n = 2
     current = np.array([1], np.int64)
     # per-level state arrays are kept in flat form after each level
@@ -82,8 +74,6 @@ def build_tree(case):
     SZ = np.concatenate(size)
     LF = np.concatenate(leaf)
     CU = np.concatenate(can_unsplit)
-    SD = np.concatenate(sdist)
-    NR = np.concatenate(nearest)
     BCA = np.zeros((2, 2 * D), np.uint8)
     ACT = np.array([False, True])
     inflow_r = BC_TYPES["inflow_r"]
@@ -91,12 +81,6 @@ def build_tree(case):
         c = CT[current]
         s = SZ[current]
         lev = LV[current].astype(np.int64)
-        # compute_signed_distances (scene.cpp:47-85)
-        sd, near = _signed_distance(case, c)
-        SD[current] = sd
-        NR[current] = near
-        if case.geometry:
-            ACT[current] = True  # fluid objects carry no boundary condition
         # compute_node_boundaries (scene.cpp:88-147): domain faces only
         has_open = np.zeros(current.size, bool)
         bcl = np.zeros((current.size, 2 * D), np.uint8)
@@ -118,10 +102,6 @@ def build_tree(case):
         m = lev < case.min_refinement_level
         ss |= m
         cu[m] = False
-        if case.geometry:
-            m = (np.abs(sd) < s * case.refinement_radius) & (lev < case.max_refinement_level)
-            ss |= m
-            cu[m] = True
         for f in range(2 * D):
             if case.bcs[f].type == inflow_r:
                 m = (bcl[:, f] != 0) & (lev < case.max_refinement_level)
@@ -156,14 +136,11 @@ def build_tree(case):
         SZ = np.concatenate([SZ, np.repeat(0.5 * ps, NCH)])
         LF = np.concatenate([LF, np.ones(m_new, bool)])
         CU = np.concatenate([CU, np.repeat(CU[sp], NCH)])
-        SD = np.concatenate([SD, np.zeros(m_new)])
-        NR = np.concatenate([NR, np.zeros(m_new, np.int64)])
         BCA = np.concatenate([BCA, np.zeros((m_new, 2 * D), np.uint8)])
         ACT = np.concatenate([ACT, np.ones(m_new, bool)])
         current = np.arange(n, n + m_new, dtype=np.int64)
         n += m_new
-    if not case.geometry:
-        ACT[1:] = np.where(LF[1:], True, ACT[1:])
+    ACT[1:] = np.where(LF[1:], True, ACT[1:])
     # fix_boundary (scene.cpp:250-335) only changes trees with solid objects.
     # cells outside the active box (scene.cpp:407-415)
     outside = np.zeros(n, bool)
@@ -174,12 +151,9 @@ def build_tree(case):
     p = np.zeros((n, NE))
     al = np.nonzero(ACT & LF)[0]
     p[al] = initial_state(case, CT[al])
-    if case.model == MODEL_MULTI and D > 1 and case.geometry:
-        _init_interface(case, p, al, SD, NR, NE)
     flags = (LF.astype(np.uint8) * F_LEAF) | (ACT.astype(np.uint8) * F_ACTIVE) | np.uint8(F_CAN_SPLIT) | \
         (CU.astype(np.uint8) * F_CAN_UNSPLIT)
     flags[0] = 0
-    _ = F_SHOULD_SPLIT, F_BOUNDARY
     return Tree(n=n, parent=np.ascontiguousarray(P, np.int32), child_base=np.ascontiguousarray(CB, np.int32),
                 level=np.ascontiguousarray(LV, np.int8), flags=np.ascontiguousarray(flags, np.uint8),
                 bc=np.ascontiguousarray(BCA, np.uint8), center=np.ascontiguousarray(CT),
@@ -208,19 +182,38 @@ def initial_state(case, x):
     raise ValueError(case.ic_type)
 
 
-def _init_interface(case, p, leaves, SD, NR, NE):
-    """Scene::init_interface (scene.cpp:340-366); diffuse_ic is never read
-    from the TOML (config.cpp:198-199), so the interface is sharp."""
-    rho = case.rho_ref
-    for mi, g in enumerate(case.geometry):
-        sel = leaves[NR[leaves] == mi]
-        sd = SD[sel]
-        if g.type == "droplet":
-            alpha = np.where(sd < 0, 1.0, 0.0)
-        elif g.type == "bubble":
-            alpha = np.where(sd > 0, 1.0, 0.0)
-        else:
-            continue
-        p[sel, 0] = rho[0] * alpha
-        p[sel, 1] = rho[1] * (1.0 - alpha)
-        p[sel, NE - 1] = alpha
+def _build_tree_cpp(case):
+    """The tree of a case with objects, from its TOML text, through
+    dacti_scene_build (include/dacti_gpu.h)."""
+    text = getattr(case, "toml_text", None)
+    if text is None:
+        raise ValueError("a case with [[geometry]] objects is built from its TOML text (load_case / loads_case)")
+    lib = _lib.load()
+    grid = shape = None
+    if case.ic_type == "grid":
+        g = np.ascontiguousarray(case.ic_grid, np.float64)
+        grid = g.ctypes.data
+        shape = (C.c_int32 * 3)(*(list(g.shape[:case.dim]) + [0] * (3 - case.dim)))
+    h = C.c_void_p()
+    rc = lib.dacti_scene_build(text.encode(), str(case.base_dir).encode(), case.model, case.dim, case.limiter, grid,
+                               shape, C.byref(h))
+    if rc != 0:
+        raise ValueError(lib.dacti_last_error(None).decode())
+    try:
+        t = _lib.TreeStruct()
+        lib.dacti_scene_tree(h, C.byref(t))
+        n, D, NE = t.n_nodes, case.dim, case.num_eqns
+
+        def arr(ptr, dtype, cols=1):
+            a = np.ctypeslib.as_array(ptr, shape=(n * cols,)).copy().astype(dtype, copy=False)
+            return a.reshape(n, cols) if cols > 1 else a
+
+        recs = np.zeros(t.n_records, RECORD_DTYPE)
+        if t.n_records:
+            C.memmove(recs.ctypes.data, t.records, RECORD_DTYPE.itemsize * t.n_records)
+        return Tree(n=n, parent=arr(t.parent, np.int32), child_base=arr(t.child_base, np.int32),
+                    level=arr(t.tree_level, np.int8), flags=arr(t.flags, np.uint8), bc=arr(t.bc, np.uint8, 2 * D),
+                    center=arr(t.center, np.float64, D), size=arr(t.size, np.float64),
+                    p=arr(t.p, np.float64, NE), brec=arr(t.brec, np.int32), records=recs)
+    finally:
+        lib.dacti_scene_destroy(h)

@@ -13,8 +13,11 @@
 // marked.  Scope as in the Python loader (dacti_amd/case.py,
 // dacti_amd/scene.py), which this header reproduces bit for bit: domain
 // boundary conditions, periodic axes, uniform and jump initial conditions,
-// analytic fluid objects (circle, sphere) of type "bubble" or "droplet",
-// axis-aligned refinement zones.  Solid geometry raises an error.
+// the objects of [[geometry]] (built-in 2D shapes and STL meshes, see
+// scene/geometry.hpp; an analytic "sphere" of `radius` is an extension of
+// this build), fluid objects ("bubble", "droplet") and solids with their
+// boundary conditions, axis-aligned refinement zones.  Refinement zones
+// with transform_like keep their axis-aligned box.
 #ifndef DACTI_CASE_HPP
 #define DACTI_CASE_HPP
 
@@ -22,11 +25,14 @@
 #include <array>
 #include <cmath>
 #include <limits>
+#include <filesystem>
 #include <map>
+#include <memory>
 #include <string>
 #include <vector>
 
 #include "../../dacti_gpu.h"
+#include "../scene/geometry.hpp"
 #include "toml.hpp"
 
 namespace dacti {
@@ -41,10 +47,17 @@ struct BC {
     double fade_in = 0.0;
 };
 
-struct Geometry {  // analytic fluid object: circle (2D) or sphere (3D)
-    std::string name, type, shape;
+// An object of the scene (config/config.hpp:69-77): one connected component
+// of a geometry's mesh, or an analytic sphere, with the geometry's type
+// ("solid" by default, "droplet", "bubble", ...) and boundary condition.
+struct Object {
+    std::string name, type = "solid", shape;
+    bool has_bc = false;
+    BC bc;
+    bool analytic = false;  // sphere of `radius` around `center`
     std::array<double, 3> center{0, 0, 0};
     double radius = 0;
+    std::shared_ptr<geom::Mesh> mesh;
 };
 
 struct Zone {
@@ -78,7 +91,7 @@ struct Case {
     // [shape[0]][shape[1]]([shape[2]])[num_eqns]
     std::vector<double> ic_grid;
     std::array<int, 3> ic_grid_shape{0, 0, 0};
-    std::vector<Geometry> geometry;
+    std::vector<Object> objects;
     std::vector<Zone> zones;
 
     int num_eqns() const { return model == DACTI_MODEL_MULTI ? dim + 4 : dim + 2; }
@@ -288,7 +301,8 @@ inline int limiter_id(const std::string &name) {
 }
 
 // Config<M>::load_config on a parsed table (config/config.cpp:10-141)
-inline Case load_case(const toml::Value &tbl, int model, int dim, int limiter = DACTI_LIMITER_MINMOD) {
+inline Case load_case(const toml::Value &tbl, int model, int dim, int limiter = DACTI_LIMITER_MINMOD,
+                      const std::string &base_dir = ".") {
     using namespace detail;
     Case cs;
     cs.model = model;
@@ -348,21 +362,94 @@ inline Case load_case(const toml::Value &tbl, int model, int dim, int limiter = 
             cs.bcs[2 * d + 1] = parse_bc(cs, sub ? sub->get("pos") : nullptr);
         }
     }
-    // geometry: config/config.cpp:66-125 (built-in analytic shapes of fluid objects)
+    // geometry: config/config.cpp:62-125, scene/geometry_data.hpp:34-146
     if (const toml::Value *geo = tbl.get("geometry")) {
+        const std::vector<double> z3{0.0, 0.0, 0.0}, o3{1.0, 1.0, 1.0};
         for (const auto &g : geo->a) {
+            const std::string name = get_str(*g, "name");
             const std::string gtype = get_str(*g, "type", "solid");
-            if (gtype == "solid" || g->get("path"))
-                throw ConfigError("solid / mesh geometry is not supported by this engine");
-            const std::vector<double> z3{0.0, 0.0, 0.0};
-            const std::vector<double> tr = get_vec(*g, "translation", 3, &z3);
-            Geometry ge;
-            ge.name = g->get("name") ? get_str(*g, "name") : "";
-            ge.type = gtype;
-            ge.shape = get_str(*g, "shape");
-            ge.center = {tr[0], tr[1], tr[2]};
-            ge.radius = get_float(*g, "radius");
-            cs.geometry.push_back(ge);
+            geom::Transform tf;
+            const auto rot = get_vec(*g, "rotation", 3, &z3), sc = get_vec(*g, "scaling", 3, &o3),
+                       tr = get_vec(*g, "translation", 3, &z3);
+            tf.rotation = vec3_t(rot[0], rot[1], rot[2]);
+            tf.scaling = vec3_t(sc[0], sc[1], sc[2]);
+            tf.translation = vec3_t(tr[0], tr[1], tr[2]);
+            std::vector<std::shared_ptr<geom::Mesh>> meshes;
+            Object base;
+            base.name = name;
+            base.type = gtype;
+            if (g->get("path")) {
+                std::filesystem::path gp = std::filesystem::path(base_dir) / get_str(*g, "path");
+                bool smooth = true;
+                if (const toml::Value *sm = g->get("smooth")) {
+                    if (sm->kind != toml::Value::BOOLEAN) throw ConfigError("Config: Type of smooth should be bool");
+                    smooth = sm->b;
+                }
+                for (auto &m : geom::stl_meshes(gp.string(), tf)) {
+                    m.is_smooth = smooth;
+                    meshes.push_back(std::make_shared<geom::Mesh>(std::move(m)));
+                }
+                base.shape = "mesh";
+            } else if (g->get("shape")) {
+                const std::string shape = get_str(*g, "shape");
+                base.shape = shape;
+                auto v2 = [&](const char *k) {
+                    const auto v = get_vec(*g, k, 2);
+                    return vec2_t(v[0], v[1]);
+                };
+                geom::Mesh m;
+                bool have = true;
+                if (shape == "ellipse") {
+                    m = geom::curve_mesh(geom::ellipse(get_float(*g, "a"), get_float(*g, "b")), 0.0, 2 * M_PI, true, tf);
+                } else if (shape == "circle") {
+                    m = geom::curve_mesh(geom::circle(get_float(*g, "radius")), 0.0, 2 * M_PI, true, tf);
+                } else if (shape == "naca_airfoil") {
+                    m = geom::curve_mesh(geom::naca4(get_float(*g, "camber"), get_float(*g, "position"),
+                                                     get_float(*g, "thickness")), -1.0, 1.0, false, tf);
+                } else if (shape == "Joukowski_airfoil") {
+                    const double zero = 0.0;
+                    m = geom::curve_mesh(geom::joukowski(get_float(*g, "camber"), get_float(*g, "thickness"),
+                                                         get_float(*g, "trailing_edge_angle", &zero)),
+                                         0.0, 2 * M_PI, true, tf);
+                } else if (shape == "line") {
+                    const double zero = 0.0, len = get_float(*g, "length", &zero);
+                    m = len > 0.0 ? geom::polygon_mesh({vec2_t(-len / 2.0, 0), vec2_t(len / 2.0, 0)}, tf)
+                                  : geom::polygon_mesh({v2("v1"), v2("v2")}, tf);
+                } else if (shape == "diamond") {
+                    const double a = get_float(*g, "width"), b = get_float(*g, "height");
+                    m = geom::polygon_mesh({vec2_t(0, b / 2), vec2_t(a / 2, 0), vec2_t(0, -b / 2), vec2_t(-a / 2, 0)}, tf);
+                } else if (shape == "quadrilateral") {
+                    m = geom::polygon_mesh({v2("v1"), v2("v2"), v2("v3"), v2("v4")}, tf);
+                } else if (shape == "triangle") {
+                    m = geom::polygon_mesh({v2("v1"), v2("v2"), v2("v3")}, tf);
+                } else if (shape == "sphere") {
+                    have = false;
+                    base.analytic = true;
+                    base.radius = get_float(*g, "radius");
+                    base.center = {tr[0], tr[1], tr[2]};
+                } else {
+                    throw ConfigError("Unknown shape type " + shape);
+                }
+                if (have) meshes.push_back(std::make_shared<geom::Mesh>(std::move(m)));
+            } else {
+                throw ConfigError("Geometry '" + name + "' needs to either have an associated file path or a shape.");
+            }
+            const toml::Value *gbc = bct ? bct->get(name) : nullptr;
+            if (gtype == "solid" && !gbc)
+                throw ConfigError("Must define boundary condition for solid geometry '" + name + "'");
+            if (gbc) {
+                base.has_bc = true;
+                base.bc = parse_bc(cs, gbc);
+            }
+            if (base.analytic) {
+                cs.objects.push_back(base);
+            } else {
+                for (auto &m : meshes) {
+                    Object o = base;
+                    o.mesh = m;
+                    cs.objects.push_back(o);
+                }
+            }
         }
     }
     if (const toml::Value *zs = tbl.get("refinement_zone")) {
@@ -375,7 +462,8 @@ inline Case load_case(const toml::Value &tbl, int model, int dim, int limiter = 
 }
 
 inline Case load_case_file(const std::string &path, int model, int dim, int limiter = DACTI_LIMITER_MINMOD) {
-    return load_case(*toml::parse_file(path), model, dim, limiter);
+    const std::string dir = std::filesystem::absolute(path).parent_path().string();
+    return load_case(*toml::parse_file(path), model, dim, limiter, dir);
 }
 
 // the C-ABI case structure (dacti_create)
@@ -418,18 +506,22 @@ inline dacti_case to_dacti_case(const Case &cs) {
     return c;
 }
 
-// Initial tree (the layout of dacti_tree), owning its arrays.
+// Initial tree (the layout of dacti_tree), owning its arrays.  Faces with an
+// object boundary carry bc code DACTI_BC_OBJECT; their condition is the
+// node's record brec (the nearest object boundary of the node's centre,
+// scene/scene.cpp:11-28), shared by all the node's object faces.
 struct Tree {
     int64_t n = 0;
     int dim = 2, num_eqns = 4;
-    std::vector<int32_t> parent, child_base;
+    std::vector<int32_t> parent, child_base, brec;
     std::vector<int8_t> level;
     std::vector<uint8_t> flags, bc;
     std::vector<double> center, size, p;
+    std::vector<dacti_boundary> records;
 
     dacti_tree view() const {
         return {n, parent.data(), child_base.data(), level.data(), flags.data(), bc.data(), center.data(), size.data(),
-                p.data()};
+                p.data(), brec.data(), (int64_t) records.size(), records.data()};
     }
 };
 
@@ -456,42 +548,174 @@ inline std::vector<double> initial_state(const Case &cs, const double *x) {
     throw ConfigError("unsupported initial condition " + cs.ic_type);
 }
 
+namespace detail {
+
+// signed distance of x (dim components) to an object and, when asked, the
+// nearest point's normal (unit, dim components)
+inline double object_distance(const Object &o, const double *x, int dim, double *normal = nullptr,
+                              double *abs_dist = nullptr) {
+    if (o.analytic) {
+        double r2 = 0.0;
+        for (int d = 0; d < dim; ++d) {
+            const double dx = x[d] - o.center[d];
+            r2 += dx * dx;
+        }
+        const double r = std::sqrt(r2), sd = r - o.radius;
+        if (normal)
+            for (int d = 0; d < dim; ++d) normal[d] = r > 0 ? (x[d] - o.center[d]) / r : (d == 0 ? 1.0 : 0.0);
+        if (abs_dist) *abs_dist = std::abs(sd);
+        return sd;
+    }
+    vec3_t q;
+    for (int d = 0; d < dim; ++d) q[d] = x[d];
+    geom::Closest c;
+    const double sd = geom::signed_distance(*o.mesh, q, c);
+    if (normal) {
+        const vec3_t n = geom::normal_at(*o.mesh, c);
+        double l2 = 0;
+        for (int d = 0; d < dim; ++d) l2 += n[d] * n[d];
+        const double l = std::sqrt(l2);
+        for (int d = 0; d < dim; ++d) normal[d] = n[d] / l;
+    }
+    if (abs_dist) *abs_dist = std::sqrt(c.sq);
+    return sd;
+}
+
+// does the segment a -> b cross the object's surface?
+inline bool object_ray(const Object &o, const double *a, const double *b, int dim) {
+    if (o.analytic) {
+        double da = 0, db = 0;
+        for (int d = 0; d < dim; ++d) {
+            da += (a[d] - o.center[d]) * (a[d] - o.center[d]);
+            db += (b[d] - o.center[d]) * (b[d] - o.center[d]);
+        }
+        return (std::sqrt(da) - o.radius) * (std::sqrt(db) - o.radius) <= 0;
+    }
+    vec3_t A, B;
+    for (int d = 0; d < dim; ++d) {
+        A[d] = a[d];
+        B[d] = b[d];
+    }
+    double t;
+    return geom::intersect_segment(*o.mesh, A, B, t);
+}
+
+}  // namespace detail
+
 // Scene<M>::init_domain (scene/scene.cpp:369-431): the tree grows level by
 // level (signed distances, boundaries, split detection, splits in node
-// order); children of a split occupy consecutive indices.
+// order; children of a split occupy consecutive indices), then
+// fix_boundary runs ten times (:250-335), cells outside the active box are
+// deactivated and the active leaves get the initial state (with the
+// interface of droplet and bubble objects, :340-366).  Where the reference
+// reads neighbours' activity while other threads change it
+// (compute_node_boundaries, :88-147), this reads the values from before the
+// pass.
 inline Tree build_tree(const Case &cs) {
     const int D = cs.dim, NE = cs.num_eqns(), NCH = 1 << D, NF = 2 * D;
+    const bool has_obj = !cs.objects.empty();
     Tree T;
     T.dim = D;
     T.num_eqns = NE;
-    std::vector<int32_t> &P = T.parent, &CB = T.child_base;
+    std::vector<int32_t> &P = T.parent, &CB = T.child_base, &BR = T.brec;
     std::vector<int8_t> &LV = T.level;
     std::vector<double> CT((size_t) 2 * D, 0.0), SZ{0.0, cs.side_length}, SD{-1.0, 0.0};
-    std::vector<char> LF{0, 1}, CU{0, 0}, ACT{0, 1};
+    std::vector<char> LF{0, 1}, CU{0, 0}, ACT{0, 1}, ISB{0, 0};
     std::vector<int64_t> NR{0, 0};
     std::vector<uint8_t> BCA((size_t) 2 * NF, 0);
     P = {0, 0};
     CB = {0, 0};
     LV = {-1, 0};
+    BR = {-1, -1};
     int64_t n = 2;
+
+    // Domain::find (integrator/domain.hpp:65-82,407-417)
+    auto find = [&](std::vector<double> x, int max_level) -> int64_t {
+        for (int d = 0; d < D; ++d) {
+            if (x[(size_t) d] < -0.5 * cs.active_size[d] || x[(size_t) d] > 0.5 * cs.active_size[d]) {
+                if (!cs.periodic[d]) return 0;
+                x[(size_t) d] += x[(size_t) d] < -0.5 * cs.active_size[d] ? cs.active_size[d] : -cs.active_size[d];
+            }
+        }
+        for (int d = 0; d < D; ++d) x[(size_t) d] = 2.0 * x[(size_t) d] / SZ[1];
+        int64_t idx = 1;
+        while (!LF[(size_t) idx] && LV[(size_t) idx] != max_level) {
+            int q = 0;
+            for (int d = 0; d < D; ++d)
+                if (x[(size_t) d] > 0.0) q |= 1 << d;
+            for (int d = 0; d < D; ++d) x[(size_t) d] = (x[(size_t) d] - 0.5 * (((q >> d) & 1) ? 1.0 : -1.0)) * 2.0;
+            idx = CB[(size_t) idx] + q;
+        }
+        return idx;
+    };
+    auto nb_center = [&](int64_t i, int f) {
+        std::vector<double> x(&CT[(size_t) i * D], &CT[(size_t) i * D] + D);
+        x[(size_t) (f >> 1)] += (2 * (f & 1) - 1) * SZ[(size_t) i];
+        return x;
+    };
+    // Scene::raycast (:31-45): false at the first object without a boundary condition
+    auto raycast = [&](int64_t i, const std::vector<double> &b) {
+        for (const Object &o : cs.objects) {
+            if (!o.has_bc) return false;
+            if (detail::object_ray(o, &CT[(size_t) i * D], b.data(), D)) return true;
+        }
+        return false;
+    };
+    // Scene::nearest_boundary (:11-28): the condition of the nearest object
+    // with one, with its unit normal there; a new record
+    auto nearest_boundary = [&](int64_t i) {
+        dacti_boundary r{};
+        r.type = DACTI_BC_ZERO_GRADIENT;
+        double best = std::numeric_limits<double>::max();
+        for (const Object &o : cs.objects) {
+            if (!o.has_bc) continue;
+            double nrm[3] = {0, 0, 0}, dist = 0;
+            detail::object_distance(o, &CT[(size_t) i * D], D, nrm, &dist);
+            if (dist < best) {
+                best = dist;
+                r = dacti_boundary{};
+                r.type = o.bc.type;
+                for (int d = 0; d < D; ++d) r.normal[d] = nrm[d];
+                for (size_t e = 0; e < o.bc.value.size() && e < 8; ++e) r.p[e] = o.bc.value[e];
+                r.fade_in = o.bc.fade_in;
+            }
+        }
+        T.records.push_back(r);
+        return (int32_t) T.records.size() - 1;
+    };
+    // Domain::split (domain.hpp:192-247) of a leaf: 2^D consecutive children
+    auto split = [&](int64_t par) {
+        const int64_t base = n;
+        CB[(size_t) par] = (int32_t) base;
+        LF[(size_t) par] = 0;
+        const double ps = SZ[(size_t) par], q = 0.25 * ps;
+        for (int ch = 0; ch < NCH; ++ch) {
+            P.push_back((int32_t) par);
+            CB.push_back(0);
+            LV.push_back((int8_t) (LV[(size_t) par] + 1));
+            for (int d = 0; d < D; ++d) CT.push_back(CT[(size_t) par * D + d] + (((ch >> d) & 1) ? 1.0 : -1.0) * q);
+            SZ.push_back(0.5 * ps);
+            LF.push_back(1);
+            CU.push_back(1);
+            SD.push_back(0.0);
+            NR.push_back(0);
+            for (int f = 0; f < NF; ++f) BCA.push_back(0);
+            ACT.push_back(1);
+            ISB.push_back(0);
+            BR.push_back(-1);
+            ++n;
+        }
+        return base;
+    };
+
     std::vector<int64_t> current{1};
     while (!current.empty()) {
-        std::vector<char> ss(current.size(), 0);
-        for (size_t k = 0; k < current.size(); ++k) {
-            const int64_t i = current[k];
-            const double *c = &CT[(size_t) i * D];
-            const double s = SZ[(size_t) i];
-            const int lev = LV[(size_t) i];
-            // compute_signed_distances (scene.cpp:47-85), analytic objects
+        // compute_signed_distances (scene.cpp:47-85)
+        for (int64_t i : current) {
             double sd = std::numeric_limits<double>::max();
             int64_t near = 0;
-            for (size_t mi = 0; mi < cs.geometry.size(); ++mi) {
-                double r2 = 0.0;
-                for (int d = 0; d < D; ++d) {
-                    const double dx = c[d] - cs.geometry[mi].center[d];
-                    r2 += dx * dx;
-                }
-                const double dist = std::sqrt(r2) - cs.geometry[mi].radius;
+            for (size_t mi = 0; mi < cs.objects.size(); ++mi) {
+                const double dist = detail::object_distance(cs.objects[mi], &CT[(size_t) i * D], D);
                 if (dist < sd) {
                     sd = dist;
                     near = (int64_t) mi;
@@ -499,36 +723,71 @@ inline Tree build_tree(const Case &cs) {
             }
             SD[(size_t) i] = sd;
             NR[(size_t) i] = near;
-            if (!cs.geometry.empty()) ACT[(size_t) i] = 1;  // fluid objects carry no boundary condition
-            // compute_node_boundaries (scene.cpp:88-147): domain faces only
-            bool has_open = false;
-            for (int d = 0; d < D; ++d)
-                for (int dr = 0; dr < 2; ++dr) {
-                    const double sgn = (double) (2 * dr - 1);
-                    const double nc = c[d] + sgn * s;
-                    const bool on_b = !cs.periodic[d] && nc * sgn > cs.active_size[d] / 2.0;
-                    BCA[(size_t) i * NF + 2 * d + dr] = on_b ? (uint8_t) (1 + 2 * d + dr) : 0;
-                    has_open = has_open || !on_b;
+            if (has_obj) ACT[(size_t) i] = sd < 0.0 ? !cs.objects[(size_t) near].has_bc : 1;
+        }
+        // compute_node_boundaries (scene.cpp:88-147)
+        const std::vector<char> act0 = ACT;
+        for (int64_t i : current) {
+            bool open = false, obj = false;
+            for (int f = 0; f < NF; ++f) {
+                const int d = f >> 1, dir = f & 1;
+                const std::vector<double> nc = nb_center(i, f);
+                uint8_t code = 0;
+                if (!cs.periodic[d] && nc[(size_t) d] * (2 * dir - 1) > cs.active_size[d] / 2.0) {
+                    code = (uint8_t) (1 + f);
+                } else {
+                    // without objects every other face is open
+                    const int64_t nb = has_obj ? find(nc, LV[(size_t) i]) : 0;
+                    if (has_obj && (act0[(size_t) nb] != act0[(size_t) i] || raycast(i, nc))) {
+                        code = DACTI_BC_OBJECT;
+                        obj = true;
+                    } else {
+                        open = true;
+                    }
                 }
-            if (!has_open) ACT[(size_t) i] = 0;
-            // detect_splits (scene.cpp:150-214), statement order kept
-            bool split = false;
-            char cu = CU[(size_t) i];
+                BCA[(size_t) i * NF + f] = code;
+            }
+            if (!open) ACT[(size_t) i] = 0;
+            if (obj) {
+                CU[(size_t) i] = 0;
+                ISB[(size_t) i] = 1;
+                BR[(size_t) i] = nearest_boundary(i);
+            }
+        }
+        // detect_splits (scene.cpp:150-214), statement order kept
+        std::vector<int64_t> sp;
+        for (int64_t i : current) {
+            const double s = SZ[(size_t) i];
+            const int lev = LV[(size_t) i];
+            const double *c = &CT[(size_t) i * D];
+            bool split_it = false;
+            char &cu = CU[(size_t) i];
             if (lev < cs.min_refinement_level) {
-                split = true;
+                split_it = true;
                 cu = 0;
             }
-            if (!cs.geometry.empty() && std::abs(sd) < s * cs.refinement_radius && lev < cs.max_refinement_level) {
-                split = true;
-                cu = 1;
-            }
-            for (int f = 0; f < NF; ++f)
-                if (cs.bcs[f].type == DACTI_BC_INFLOW_R && BCA[(size_t) i * NF + f] && lev < cs.max_refinement_level) {
-                    split = true;
+            if (has_obj && std::abs(SD[(size_t) i]) < s * cs.refinement_radius) {
+                if (cs.objects[(size_t) NR[(size_t) i]].type == "solid") {
+                    if (lev < cs.max_refinement_level_geom) {
+                        split_it = true;
+                        cu = 0;
+                    }
+                } else if (lev < cs.max_refinement_level) {
+                    split_it = true;
                     cu = 1;
                 }
+            }
+            for (int f = 0; f < NF; ++f) {
+                const uint8_t code = BCA[(size_t) i * NF + f];
+                const int type = code == DACTI_BC_OBJECT ? T.records[(size_t) BR[(size_t) i]].type
+                                 : code ? cs.bcs[code - 1].type : -1;
+                if (type == DACTI_BC_INFLOW_R && lev < cs.max_refinement_level) {
+                    split_it = true;
+                    cu = 1;
+                }
+            }
             if (cs.ic_type == "jump" && std::abs(c[cs.ic_axis] - cs.ic_location) < s && lev < cs.max_refinement_level) {
-                split = true;
+                split_it = true;
                 cu = 1;
             }
             for (const Zone &z : cs.zones) {
@@ -536,45 +795,84 @@ inline Tree build_tree(const Case &cs) {
                 for (int d = 0; d < D; ++d) x3[d] = c[d];
                 bool inside = true;
                 for (int a = 0; a < 3; ++a) inside = inside && x3[a] < z.x1[a] && x3[a] > z.x0[a];
-                if (inside && lev < z.level) split = true;
+                if (inside && lev < z.level) split_it = true;
             }
-            CU[(size_t) i] = cu;
-            ss[k] = split;
+            if (split_it && LF[(size_t) i]) sp.push_back(i);
         }
         // split_detected_nodes (scene.cpp:217-247), parents in node order
-        std::vector<int64_t> sp;
-        for (size_t k = 0; k < current.size(); ++k)
-            if (ss[k] && LF[(size_t) current[k]]) sp.push_back(current[k]);
         std::sort(sp.begin(), sp.end());
-        if (sp.empty()) break;
         std::vector<int64_t> next;
         for (int64_t par : sp) {
-            const int64_t base = n;
-            CB[(size_t) par] = (int32_t) base;
-            LF[(size_t) par] = 0;
-            const double ps = SZ[(size_t) par], q = 0.25 * ps;
+            const int64_t base = split(par);
             for (int ch = 0; ch < NCH; ++ch) {
-                P.push_back((int32_t) par);
-                CB.push_back(0);
-                LV.push_back((int8_t) (LV[(size_t) par] + 1));
-                for (int d = 0; d < D; ++d) CT.push_back(CT[(size_t) par * D + d] + (((ch >> d) & 1) ? 1.0 : -1.0) * q);
-                SZ.push_back(0.5 * ps);
-                LF.push_back(1);
-                CU.push_back(CU[(size_t) par]);
-                SD.push_back(0.0);
-                NR.push_back(0);
-                for (int f = 0; f < NF; ++f) BCA.push_back(0);
-                ACT.push_back(1);
-                next.push_back(n);
-                ++n;
+                CU[(size_t) (base + ch)] = CU[(size_t) par];
+                next.push_back(base + ch);
             }
         }
         current.swap(next);
     }
-    if (cs.geometry.empty())
+    if (!has_obj)
         for (int64_t i = 1; i < n; ++i)
             if (LF[(size_t) i]) ACT[(size_t) i] = 1;
-    // fix_boundary (scene.cpp:250-335) only changes trees with solid objects.
+
+    // fix_boundary (scene.cpp:250-335), ten times
+    for (int it = 0; it < 10 && has_obj; ++it) {
+        std::vector<int64_t> fresh;  // new_leaves, in insertion order
+        std::vector<char> is_fresh((size_t) n, 0);
+        for (int64_t i = 0; i < n; ++i) {
+            if (ACT[(size_t) i] || !LF[(size_t) i]) continue;
+            for (int f = 0; f < NF; ++f) {
+                const std::vector<double> nc = nb_center(i, f);
+                while (true) {
+                    const int64_t nb = find(nc, LV[(size_t) i]);
+                    if (!(nb && ACT[(size_t) nb] && LV[(size_t) nb] < LV[(size_t) i])) break;
+                    const int64_t base = split(nb);
+                    is_fresh.resize((size_t) n, 0);
+                    if (is_fresh[(size_t) nb]) {
+                        is_fresh[(size_t) nb] = 0;
+                        fresh.erase(std::find(fresh.begin(), fresh.end(), nb));
+                    }
+                    for (int ch = 0; ch < NCH; ++ch) {
+                        const int64_t x = base + ch;
+                        CU[(size_t) x] = CU[(size_t) nb];
+                        for (int g = 0; g < NF; ++g) {
+                            const int a = g >> 1, dir = g & 1;
+                            if (((ch >> a) & 1) == dir) {
+                                BCA[(size_t) x * NF + g] = BCA[(size_t) nb * NF + g];
+                                if (BCA[(size_t) x * NF + g] == DACTI_BC_OBJECT) BR[(size_t) x] = BR[(size_t) nb];
+                            }
+                        }
+                        ACT[(size_t) x] = 1;
+                        is_fresh[(size_t) x] = 1;
+                        fresh.push_back(x);
+                    }
+                }
+            }
+        }
+        is_fresh.resize((size_t) n, 0);
+        for (int64_t i : fresh) {
+            int32_t rec = -1;
+            for (int f = 0; f < NF; ++f) {
+                BCA[(size_t) i * NF + f] = 0;
+                const std::vector<double> nc = nb_center(i, f);
+                const int64_t nb = find(nc, LV[(size_t) i]);
+                if (ACT[(size_t) nb] != ACT[(size_t) i] || raycast(i, nc)) {
+                    if (rec < 0) rec = nearest_boundary(i);
+                    BCA[(size_t) i * NF + f] = DACTI_BC_OBJECT;
+                }
+            }
+            BR[(size_t) i] = rec;
+        }
+        for (int64_t i = 0; i < n; ++i) {
+            if (!(ACT[(size_t) i] && LF[(size_t) i])) continue;
+            for (int f = 0; f < NF; ++f) {
+                if (!BCA[(size_t) i * NF + f]) continue;
+                const int64_t nb = find(nb_center(i, f), LV[(size_t) i]);
+                if (nb && !BCA[(size_t) nb * NF + (f ^ 1)]) BCA[(size_t) i * NF + f] = 0;
+            }
+        }
+    }
+
     // cells outside the active box (scene.cpp:407-415)
     for (int64_t i = 0; i < n; ++i)
         for (int d = 0; d < D; ++d) {
@@ -594,11 +892,11 @@ inline Tree build_tree(const Case &cs) {
             for (int e = 0; e < NE; ++e) T.p[(size_t) i * NE + e] = v[(size_t) e];
             // Scene::init_interface (scene.cpp:340-366); diffuse_ic is never
             // read from the TOML (config.cpp:198-199), so the interface is sharp
-            if (cs.model == DACTI_MODEL_MULTI && D > 1 && !cs.geometry.empty()) {
-                const Geometry &g = cs.geometry[(size_t) NR[(size_t) i]];
+            if (cs.model == DACTI_MODEL_MULTI && D > 1 && has_obj) {
+                const Object &o = cs.objects[(size_t) NR[(size_t) i]];
                 const double sd = SD[(size_t) i];
-                if (g.type == "droplet" || g.type == "bubble") {
-                    const double alpha = g.type == "droplet" ? (sd < 0 ? 1.0 : 0.0) : (sd > 0 ? 1.0 : 0.0);
+                if (o.type == "droplet" || o.type == "bubble") {
+                    const double alpha = o.type == "droplet" ? (sd < 0 ? 1.0 : 0.0) : (sd > 0 ? 1.0 : 0.0);
                     T.p[(size_t) i * NE + 0] = cs.rho_ref[0] * alpha;
                     T.p[(size_t) i * NE + 1] = cs.rho_ref[1] * (1.0 - alpha);
                     T.p[(size_t) i * NE + NE - 1] = alpha;
@@ -606,7 +904,8 @@ inline Tree build_tree(const Case &cs) {
             }
         }
         T.flags[(size_t) i] = (uint8_t) ((LF[(size_t) i] ? DACTI_F_LEAF : 0) | (ACT[(size_t) i] ? DACTI_F_ACTIVE : 0) |
-                                         DACTI_F_CAN_SPLIT | (CU[(size_t) i] ? DACTI_F_CAN_UNSPLIT : 0));
+                                         DACTI_F_CAN_SPLIT | (CU[(size_t) i] ? DACTI_F_CAN_UNSPLIT : 0) |
+                                         (ISB[(size_t) i] ? DACTI_F_BOUNDARY : 0));
     }
     return T;
 }

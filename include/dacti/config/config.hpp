@@ -28,8 +28,9 @@
 
 namespace dacti {
 
-// a polygonal object outline (2D: vertex pairs of the extruded outline,
-// rows 2i and 2i+1 share x and y), the part of ObjMesh the observers read
+// the vertices of an object's mesh, the part of ObjMesh the observers read
+// (a 2D outline extruded to z = -1, +1: rows 2i and 2i+1 share x and y;
+// empty for an analytic sphere)
 struct ObjMesh {
     struct Vertices {
         std::vector<vec3_t> rows_;
@@ -163,25 +164,15 @@ private:
         for (const Zone &z : cs.zones)
             refine_zones.push_back({vec3_t(z.x0[0], z.x0[1], z.x0[2]), vec3_t(z.x1[0], z.x1[1], z.x1[2]),
                                     (size_t) z.level});
-        for (const Geometry &g : cs.geometry) objects.push_back({outline(g), g.type, false});
+        for (const auto &o : cs.objects) {
+            ObjMesh m;
+            if (o.mesh) m.V.rows_ = o.mesh->V;
+            objects.push_back({m, o.type, o.has_bc});
+        }
         set_model_constants();
         // output directory (config/config.cpp:28-36)
         result_out_dir = (prj_dir / "out" / case_name).string();
         std::filesystem::create_directories(result_out_dir);
-    }
-
-    // a circle's outline for observers (64 segments); spheres carry none
-    static ObjMesh outline(const Geometry &g) {
-        ObjMesh m;
-        if (g.shape != "circle") return m;
-        const int n = 64;
-        for (int i = 0; i < n; ++i) {
-            const double a = 2 * M_PI * i / n;
-            const vec3_t p(g.center[0] + g.radius * std::cos(a), g.center[1] + g.radius * std::sin(a), 0.0);
-            m.V.rows_.push_back(p);
-            m.V.rows_.push_back(p);
-        }
-        return m;
     }
 
     void set_model_constants() {

@@ -52,6 +52,8 @@ typedef struct dacti_handle dacti_handle;
 #define DACTI_BC_INFLOW_R 4
 #define DACTI_BC_OUTFLOW 5
 #define DACTI_BC_DIRICHLET 6
+/* bc code of a face whose condition is its node's object record (brec) */
+#define DACTI_BC_OBJECT 7
 
 /* node flag bits (tree exchange format) */
 #define DACTI_F_LEAF 1
@@ -99,10 +101,20 @@ typedef struct dacti_case {
  * (scene/scene.cpp:369-431), in the layout of integrator/domain.hpp:14-63.
  * Node 0 is the null node, node 1 the root.  A node's 2^dim children occupy
  * indices child_base .. child_base + 2^dim - 1 (0 = no children).
- * bc[i*2*dim + f] is 0 for no boundary or 1 + the domain face index of the
- * boundary condition that face carries; p holds the primitive initial state
- * of active leaves (num_eqns values per node).
+ * bc[i*2*dim + f] is 0 for no boundary, 1 + the domain face index of the
+ * boundary condition that face carries, or DACTI_BC_OBJECT for an object
+ * boundary (the record brec[i]); p holds the primitive initial state of
+ * active leaves (num_eqns values per node).  An object boundary condition
+ * (integrator/boundary.hpp:13-60) is a type (DACTI_BC_*), the unit wall
+ * normal for freeslip, the inflow state and fade-in time.
  */
+typedef struct dacti_boundary {
+    int32_t type, pad;
+    double normal[3];
+    double p[8];
+    double fade_in;
+} dacti_boundary;
+
 typedef struct dacti_tree {
     int64_t n_nodes;
     const int32_t *parent;
@@ -113,6 +125,11 @@ typedef struct dacti_tree {
     const double *center; /* n_nodes * dim */
     const double *size;
     const double *p;      /* n_nodes * num_eqns */
+    /* object boundaries (NULL / 0 without objects): brec[i] is the record
+     * of node i's DACTI_BC_OBJECT faces, -1 for none */
+    const int32_t *brec;
+    int64_t n_records;
+    const dacti_boundary *records;
 } dacti_tree;
 
 /*
@@ -129,6 +146,23 @@ typedef struct dacti_node_view {
     uint8_t *flags;
     double *center, *size, *t, *p, *u, *slope, *rhs, *k, *grad, *meta;
 } dacti_node_view;
+
+/*
+ * Case reader and initial tree (Config<M>::load_config,
+ * config/config.cpp:10-243, and Scene<M>::init_domain,
+ * scene/scene.cpp:369-431): reads the TOML text of a case (geometry paths
+ * relative to base_dir) for a model, dimension and limiter and builds the
+ * initial tree.  A "grid" initial condition (an extension) takes
+ * ic_grid[shape[0]][shape[1]]([shape[2]])[num_eqns].  The case and tree
+ * views stay valid until dacti_scene_destroy; errors go to
+ * dacti_last_error(NULL).
+ */
+typedef struct dacti_scene dacti_scene;
+int dacti_scene_build(const char *toml_text, const char *base_dir, int32_t model, int32_t dim, int32_t limiter,
+                      const double *ic_grid, const int32_t *ic_grid_shape, dacti_scene **out);
+int dacti_scene_case(const dacti_scene *s, dacti_case *out);
+int dacti_scene_tree(const dacti_scene *s, dacti_tree *out);
+int dacti_scene_destroy(dacti_scene *s);
 
 /* counters and timers of a handle */
 typedef struct dacti_stats {

@@ -82,6 +82,11 @@ def params(case):
     return P
 
 
+# object boundary record (dacti_boundary of include/dacti_gpu.h)
+RECORD_DTYPE = np.dtype([("type", np.int32), ("pad", np.int32), ("normal", np.float64, 3), ("p", np.float64, 8),
+                         ("fade_in", np.float64)])
+
+
 def _p(a, t):
     return a.ctypes.data_as(C.POINTER(t))
 
@@ -97,12 +102,18 @@ class Oracle:
         self.P = params(case)
         f = getattr(L, self.pfx + "create")
         f.restype = C.c_void_p
-        f.argtypes = [C.POINTER(OParams), C.c_int64] + [C.c_void_p] * 8
+        f.argtypes = [C.POINTER(OParams), C.c_int64] + [C.c_void_p] * 9 + [C.c_int64, C.c_void_p]
         arrs = [np.ascontiguousarray(tree.parent, np.int32), np.ascontiguousarray(tree.child_base, np.int32),
                 np.ascontiguousarray(tree.level, np.int8), np.ascontiguousarray(tree.flags, np.uint8),
                 np.ascontiguousarray(tree.bc, np.uint8), np.ascontiguousarray(tree.center, np.float64),
                 np.ascontiguousarray(tree.size, np.float64), np.ascontiguousarray(tree.p, np.float64)]
-        self.h = f(C.byref(self.P), tree.n, *[a.ctypes.data for a in arrs])
+        brec = getattr(tree, "brec", None)
+        brec = np.full(tree.n, -1, np.int32) if brec is None else np.ascontiguousarray(brec, np.int32)
+        recs = getattr(tree, "records", None)
+        recs = np.zeros(0, RECORD_DTYPE) if recs is None else np.ascontiguousarray(recs, RECORD_DTYPE)
+        self._keep = (arrs, brec, recs)
+        self.h = f(C.byref(self.P), tree.n, *[a.ctypes.data for a in arrs], brec.ctypes.data, len(recs),
+                   recs.ctypes.data if len(recs) else None)
         self._fn("integrate", C.c_double, [C.c_void_p, C.c_int, C.POINTER(C.c_int)])
         self._fn("run", C.c_int64, [C.c_void_p, C.c_int64, C.c_int])
         for nm, rt in (("n_nodes", C.c_int64), ("cell_updates", C.c_int64), ("n_substeps", C.c_int64),

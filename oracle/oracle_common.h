@@ -38,6 +38,8 @@ extern "C" {
 #define O_BC_INFLOW_R 4
 #define O_BC_OUTFLOW 5
 #define O_BC_DIRICHLET 6
+/* bc code of a face whose condition is its node's object record */
+#define O_BC_OBJECT 7
 
 /* node flag bits of the tree exchange format */
 #define O_F_LEAF 1

@@ -105,7 +105,8 @@ typedef struct {
     double cov[NFACE], acc_cov[NFACE];
     int32_t parent, child_base;
     int32_t nbr[NFACE];
-    uint8_t bc[NFACE]; /* 0 = none, else 1 + domain face index */
+    uint8_t bc[NFACE]; /* 0 = none, 1 + domain face index, or O_BC_OBJECT (record brec) */
+    int32_t brec;      /* object boundary record of the O_BC_OBJECT faces */
     int8_t acti, tree_level, cluster, buffer;
     uint8_t leaf, active, garbage, can_split, can_unsplit, should_split, is_boundary;
     double error, creation_time, max_dt, cfl;
@@ -129,6 +130,15 @@ typedef struct {
     int32_t *list;   /* current cluster list */
     double *tmp;     /* smoothing scratch */
     uint8_t *mark;   /* refinement marks */
+    /* object boundary conditions (scene/scene.cpp:11-28), one record per
+     * boundary node: type, unit wall normal, inflow state, fade-in */
+    struct o_rec {
+        int32_t type, pad;
+        double normal[3];
+        double p[8];
+        double fade_in;
+    } *recs;
+    int64_t n_recs;
 } o_state;
 
 /* ------------------------------------------------------------------ */
@@ -492,17 +502,50 @@ static void bc_normal(int face, double *n) {
     n[dim] = -(2.0 * dir - 1.0); /* -unit(dim)*(2dir-1): inward normal */
 }
 
-/* boundary_value: boundary.hpp:75-123.  `face` is the domain face index
- * 2*dim+dir the boundary was created for. */
-static void boundary_value(const o_params *P, int face, const o_cell *c, double *pb) {
-    int type = P->bc_type[face];
+/* A boundary condition as Boundary<M> holds it (boundary.hpp:13-60): the
+ * type, the wall normal (freeslip), the inflow state and fade-in time. */
+typedef struct {
+    int type;
+    double normal[3];
+    const double *p;
+    double fade_in;
+} o_bnd;
+
+/* the condition of domain face `face` (config/config.cpp:47-61: the wall
+ * normal of a domain face is -unit(dim)*(2dir-1)) */
+static o_bnd bnd_domain(const o_params *P, int face) {
+    o_bnd b;
+    b.type = P->bc_type[face];
+    bc_normal(face, b.normal);
+    b.p = P->bc_p[face];
+    b.fade_in = P->bc_fade_in[face];
+    return b;
+}
+
+/* the condition of face f of a node: its domain face, or its object record */
+static o_bnd bnd_of(const o_state *S, const o_node *nd, int f) {
+    const uint8_t code = nd->bc[f];
+    if (code == O_BC_OBJECT) {
+        const struct o_rec *r = &S->recs[nd->brec];
+        o_bnd b;
+        b.type = r->type;
+        for (int i = 0; i < 3; ++i) b.normal[i] = r->normal[i];
+        b.p = r->p;
+        b.fade_in = r->fade_in;
+        return b;
+    }
+    return bnd_domain(&S->P, code ? code - 1 : f);
+}
+
+/* boundary_value: boundary.hpp:75-123 */
+static void boundary_value(const o_params *P, const o_bnd *b, const o_cell *c, double *pb) {
+    int type = b->type;
     const double *pc = c->p;
     if (type == O_BC_FREESLIP) {
-        double n[3], dot = 0;
-        bc_normal(face, n);
+        double dot = 0;
         for (int i = 0; i < NE; ++i) pb[i] = pc[i];
-        for (int i = 0; i < OD; ++i) dot += pc[IU + i] * n[i];
-        for (int i = 0; i < OD; ++i) pb[IU + i] = pc[IU + i] - n[i] * dot;
+        for (int i = 0; i < OD; ++i) dot += pc[IU + i] * b->normal[i];
+        for (int i = 0; i < OD; ++i) pb[IU + i] = pc[IU + i] - b->normal[i] * dot;
         return;
     }
     if (type == O_BC_NOSLIP) {
@@ -511,9 +554,9 @@ static void boundary_value(const o_params *P, int face, const o_cell *c, double 
         return;
     }
     if (type == O_BC_INFLOW || type == O_BC_INFLOW_R) {
-        for (int i = 0; i < NE; ++i) pb[i] = P->bc_p[face][i];
+        for (int i = 0; i < NE; ++i) pb[i] = b->p[i];
         if (!supersonic(P, pc)) pb[IP] = pc[IP];
-        double fi = P->bc_fade_in[face];
+        double fi = b->fade_in;
         if (fi > 0.0) {
             double vm = smin(c->t / fi, 1.0);
             for (int i = 0; i < OD; ++i) pb[IU + i] *= vm;
@@ -521,7 +564,7 @@ static void boundary_value(const o_params *P, int face, const o_cell *c, double 
         return;
     }
     if (type == O_BC_DIRICHLET) {
-        for (int i = 0; i < NE; ++i) pb[i] = P->bc_p[face][i];
+        for (int i = 0; i < NE; ++i) pb[i] = b->p[i];
         return;
     }
     for (int i = 0; i < NE; ++i) pb[i] = pc[i]; /* outflow, zeroGradient */
@@ -529,11 +572,11 @@ static void boundary_value(const o_params *P, int face, const o_cell *c, double 
 
 /* boundary_flux: boundary.hpp:130-160 (result already scaled by
  * area * (t1 - t0) * direction) */
-static void boundary_flux(const o_params *P, int face, const o_cell *c, int dim, int dir, double area, double t0,
-                          double t1, double *out) {
+static void boundary_flux(const o_params *P, const o_bnd *b, const o_cell *c, int dim, int dir, double area,
+                          double t0, double t1, double *out) {
     double pb[NE], F[NF];
-    int type = P->bc_type[face];
-    boundary_value(P, face, c, pb);
+    int type = b->type;
+    boundary_value(P, b, c, pb);
     if (type == O_BC_FREESLIP) {
         flux_F_projected(P, c->p, pb[IU + dim], dim, F);
     } else if (type == O_BC_NOSLIP) {
@@ -822,7 +865,8 @@ static void compute_slope(o_state *S, int32_t idx) {
         int32_t in = nd->nbr[2 * dim], ip = nd->nbr[2 * dim + 1];
         double pn[NE], pp[NE], xc = 0.0, xn, xp, tn, tp;
         if (in == 0) {
-            boundary_value(&S->P, nd->bc[2 * dim] ? nd->bc[2 * dim] - 1 : 2 * dim, &nd->cell, pn);
+            o_bnd b = bnd_of(S, nd, 2 * dim);
+            boundary_value(&S->P, &b, &nd->cell, pn);
             xn = -nd->cell.size;
             tn = nd->cell.t;
         } else {
@@ -836,7 +880,8 @@ static void compute_slope(o_state *S, int32_t idx) {
             xn = -0.5 * (nd->cell.size + S->tree[in].cell.size);
         }
         if (ip == 0) {
-            boundary_value(&S->P, nd->bc[2 * dim + 1] ? nd->bc[2 * dim + 1] - 1 : 2 * dim + 1, &nd->cell, pp);
+            o_bnd b = bnd_of(S, nd, 2 * dim + 1);
+            boundary_value(&S->P, &b, &nd->cell, pp);
             xp = nd->cell.size;
             tp = nd->cell.t;
         } else {
@@ -1062,8 +1107,8 @@ static void flux_phase(o_state *S, const int32_t *list, int64_t nl, int L) {
                 }
                 if (nb == 0) {
                     double area = pow_level(ny->cell.size, OD - 1);
-                    boundary_flux(&S->P, ny->bc[f] ? ny->bc[f] - 1 : f, &ny->cell, dim, dir, area, ny->cell.t,
-                                  ny->cell.t + dt, F);
+                    o_bnd b = bnd_of(S, ny, f);
+                    boundary_flux(&S->P, &b, &ny->cell, dim, dir, area, ny->cell.t, ny->cell.t + dt, F);
                     for (int i = 0; i < NF; ++i) c[i] = c[i] - F[i];
                     cov += dt;
                 } else if (S->tree[nb].leaf) {
@@ -1175,7 +1220,8 @@ static void update_phase(o_state *S, const int32_t *list, int64_t nl, int L, int
  * impl/normal_curvature.hpp) */
 static double bc_alpha(const o_state *S, const o_node *nd, int f) {
     double pb[NE];
-    boundary_value(&S->P, nd->bc[f] ? nd->bc[f] - 1 : f, &nd->cell, pb);
+    o_bnd b = bnd_of(S, nd, f);
+    boundary_value(&S->P, &b, &nd->cell, pb);
     return pb[IA];
 }
 
@@ -1474,6 +1520,7 @@ static void adaptive_refinement(o_state *S, int L) {
                 for (int dim = 0; dim < OD; ++dim)
                     for (int dir = 0; dir < 2; ++dir)
                         if (((ci >> dim) & 1) == dir) ch->bc[2 * dim + dir] = nd->bc[2 * dim + dir];
+                ch->brec = nd->brec;
                 ch->acti = (int8_t) L;
                 ch->cluster = (int8_t) L;
             }
@@ -1574,9 +1621,14 @@ static int substep(o_state *S, int L, int signal, int *force_break) {
  * arrives in the exchange format (state after Scene::init_domain). */
 void *OPFX(create)(const o_params *P, int64_t n, const int32_t *parent, const int32_t *child_base,
                    const int8_t *level, const uint8_t *flags, const uint8_t *bc, const double *center,
-                   const double *size, const double *p) {
+                   const double *size, const double *p, const int32_t *brec, int64_t n_recs, const void *recs) {
     o_state *S = (o_state *) calloc(1, sizeof(o_state));
     S->P = *P;
+    S->n_recs = n_recs;
+    if (n_recs > 0) {
+        S->recs = (struct o_rec *) malloc(sizeof(struct o_rec) * (size_t) n_recs);
+        memcpy(S->recs, recs, sizeof(struct o_rec) * (size_t) n_recs);
+    }
     ensure_cap(S, n + 16);
     S->n = n;
     for (int64_t i = 0; i < n; ++i) {
@@ -1593,6 +1645,7 @@ void *OPFX(create)(const o_params *P, int64_t n, const int32_t *parent, const in
         nd->should_split = (flags[i] & O_F_SHOULD_SPLIT) != 0;
         nd->is_boundary = (flags[i] & O_F_BOUNDARY) != 0;
         for (int f = 0; f < NFACE; ++f) nd->bc[f] = bc[(size_t) i * NFACE + f];
+        nd->brec = brec ? brec[i] : -1;
         for (int d = 0; d < OD; ++d) nd->cell.center[d] = center[(size_t) i * OD + d];
         nd->cell.size = size[i];
         nd->cell.t = 0.0;
@@ -1655,6 +1708,7 @@ void OPFX(destroy)(void *h) {
     free(S->list);
     free(S->tmp);
     free(S->mark);
+    free(S->recs);
     free(S);
 }
 
@@ -1879,7 +1933,8 @@ void OPFX(t_boundary_flux)(const o_params *P, int face, const double *p, const d
     for (int d = 0; d < OD; ++d)
         for (int i = 0; i < NGX; ++i) c.gradient[d][i] = grad[d * NGX + i];
     c.t = t;
-    boundary_flux(P, face, &c, dim, dir, area, t0, t1, out);
+    o_bnd b = bnd_domain(P, face);
+    boundary_flux(P, &b, &c, dim, dir, area, t0, t1, out);
 }
 
 /* Direct access to single-cell model functions, for closed-form tests. */
@@ -1897,7 +1952,22 @@ void OPFX(t_boundary_value)(const o_params *P, int face, const double *p, double
     memset(&c, 0, sizeof c);
     for (int i = 0; i < NE; ++i) c.p[i] = p[i];
     c.t = t;
-    boundary_value(P, face, &c, pb);
+    o_bnd b = bnd_domain(P, face);
+    boundary_value(P, &b, &c, pb);
+}
+/* ghost state of an object boundary record (type, wall normal, inflow state, fade-in) */
+void OPFX(t_boundary_value_rec)(const o_params *P, int type, const double *normal, const double *pin, double fade,
+                                const double *p, double t, double *pb) {
+    o_cell c;
+    memset(&c, 0, sizeof c);
+    for (int i = 0; i < NE; ++i) c.p[i] = p[i];
+    c.t = t;
+    o_bnd b;
+    b.type = type;
+    for (int i = 0; i < 3; ++i) b.normal[i] = normal[i];
+    b.p = pin;
+    b.fade_in = fade;
+    boundary_value(P, &b, &c, pb);
 }
 double OPFX(t_max_delta_time)(const o_params *P, const double *p, double size) {
     o_cell c;

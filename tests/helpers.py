@@ -144,3 +144,93 @@ def ns3_box(level=3):
     case = loads_case(toml, "navier_stokes", 3)
     case.ic_grid = base.ic_grid
     return case
+
+
+# ---------------------------------------------------------------- objects
+SOLID_TOML = """
+[scene]
+name = "solid"
+domain_size = [2.0, 1.0]
+[run]
+print_interval = 1.0
+end_time = 1.0
+[integrator]
+min_refinement_level = {base}
+max_refinement_level = {top}
+max_refinement_level_geom = {geom}
+refinement_radius = 2.0
+error_threshold = [{emin}, {emax}]
+error_weights = [1.0, 0.0, 0.0, 0.1]
+max_cfl = 0.4
+[model.reference_values]
+rho = 1.0
+p = 1.0
+vel = [{u}, {v}]
+{extra}
+[initial_condition]
+type = "uniform"
+{geometry}
+[boundary_conditions.X.neg]
+type = "{xneg}"
+[boundary_conditions.X.pos]
+type = "{xpos}"
+[boundary_conditions.Y.neg]
+type = "{ybc}"
+[boundary_conditions.Y.pos]
+type = "{ybc}"
+"""
+
+SOLID_GEOMETRY = {
+    # a cylinder (noslip), the reference's cylinder case in small
+    "cylinder": """[[geometry]]
+name = "object"
+shape = "ellipse"
+a = 0.15
+b = 0.15
+translation = [-0.4, 0.0, 0.0]
+[boundary_conditions.object]
+type = "noslip"
+""",
+    # a diamond aerofoil at 8 degrees incidence (freeslip)
+    "diamond": """[[geometry]]
+name = "object"
+shape = "diamond"
+width = 0.6
+height = 0.12
+rotation = [0.0, 0.0, -8.0]
+translation = [-0.2, 0.02, 0.0]
+[boundary_conditions.object]
+type = "freeslip"
+""",
+    # two solids, a ramp and a cowl (freeslip), as in the reference's scramjet
+    "inlet": """[[geometry]]
+name = "ramp"
+shape = "triangle"
+v1 = [-1.0, -0.5]
+v2 = [0.6, -0.5]
+v3 = [0.6, -0.3]
+[[geometry]]
+name = "cowl"
+shape = "quadrilateral"
+v1 = [0.0, 0.2]
+v2 = [0.8, 0.12]
+v3 = [0.8, 0.2]
+v4 = [0.2, 0.25]
+[boundary_conditions.ramp]
+type = "freeslip"
+[boundary_conditions.cowl]
+type = "noslip"
+""",
+}
+
+
+def solid_case(kind, model="euler", base=4, top=6, geom=6, emin=0.005, emax=0.02, u=0.5, v=0.0, closed=False,
+               limiter="minmod"):
+    """A 2D case with solid objects: inflow/outflow/freeslip domain, or a
+    closed freeslip box (no mass or energy crosses any face)."""
+    extra = "Re = 200.0\nL = 1.0\nPr = 0.72\n" if model == "navier_stokes" else ""
+    text = SOLID_TOML.format(base=base, top=top, geom=geom, emin=repr(emin), emax=repr(emax), u=repr(u), v=repr(v),
+                             extra=extra, geometry=SOLID_GEOMETRY[kind],
+                             xneg="freeslip" if closed else "inflow", xpos="freeslip" if closed else "outflow",
+                             ybc="freeslip")
+    return loads_case(text, model, 2, limiter)

@@ -40,7 +40,10 @@ def test_struct_layouts_match_header():
     # dacti_case: 4 ints, 1 double, 3 doubles, 3 ints, 1 int, 4 doubles, 8 doubles, 5 + 13 doubles,
     # 6 ints, 48 doubles, 6 doubles, 24 + 24 doubles, 8 ints
     assert C.sizeof(_lib.CaseStruct) == 16 + 8 + 24 + 12 + 4 + 32 + 64 + 144 + 24 + 384 + 48 + 384 + 32
-    assert C.sizeof(_lib.TreeStruct) == 8 + 8 * 8
+    # dacti_tree: n_nodes, 8 array pointers, brec, n_records, records
+    assert C.sizeof(_lib.TreeStruct) == 8 + 8 * 8 + 8 + 8 + 8
+    # dacti_boundary: type, pad, normal[3], p[8], fade_in
+    assert _lib.RECORD_DTYPE.itemsize == 8 + 24 + 64 + 8
     assert C.sizeof(_lib.NodeViewStruct) == 17 * 8
 
 

@@ -33,8 +33,12 @@ int main(int argc, char **argv) {
     auto lev = rd<int8_t>(f, n);
     auto fl = rd<uint8_t>(f, n), bc = rd<uint8_t>(f, n * 2 * c.dim);
     auto ctr = rd<double>(f, n * c.dim), size = rd<double>(f, n), p = rd<double>(f, n * sz.num_eqns);
+    auto brec = rd<int32_t>(f, n);
+    int64_t nrec; if (fread(&nrec, 8, 1, f) != 1) return 2;
+    auto recs = rd<dacti_boundary>(f, nrec);
     std::fclose(f);
-    dacti_tree t{n, parent.data(), cb.data(), lev.data(), fl.data(), bc.data(), ctr.data(), size.data(), p.data()};
+    dacti_tree t{n, parent.data(), cb.data(), lev.data(), fl.data(), bc.data(), ctr.data(), size.data(), p.data(),
+                 brec.data(), nrec, recs.data()};
     dacti::GpuSolver g(c, t);
     bool brk = false;
     for (int s = 0; s < 2; ++s) g.integrate(brk);
@@ -62,8 +66,10 @@ def test_cpp_facade_matches_oracle(tmp_path):
         f.write(np.int64(tree.n).tobytes())
         for a, dt in ((tree.parent, np.int32), (tree.child_base, np.int32), (tree.level, np.int8),
                       (tree.flags, np.uint8), (tree.bc, np.uint8), (tree.center, np.float64),
-                      (tree.size, np.float64), (tree.p, np.float64)):
+                      (tree.size, np.float64), (tree.p, np.float64), (tree.brec, np.int32)):
             f.write(np.ascontiguousarray(a, dtype=dt).tobytes())
+        f.write(np.int64(len(tree.records)).tobytes())
+        f.write(np.ascontiguousarray(tree.records).tobytes())
     src = tmp_path / "driver.cpp"
     src.write_text(DRIVER)
     exe = tmp_path / "driver"

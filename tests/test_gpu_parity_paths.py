@@ -160,3 +160,69 @@ def test_substep_hook_follows_schedule():
     with pytest.raises(ZeroDivisionError):
         g.integrate(substep_hook=lambda *a: 1 / 0)
     g.close()
+
+
+# ------------------------------------------------------------ solid objects
+@pytest.mark.parametrize("kind,model,u", [("cylinder", "euler", 0.5), ("diamond", "euler", 2.0),
+                                          ("inlet", "euler", 1.8), ("cylinder", "navier_stokes", 0.5)])
+def test_parity_solid_objects(kind, model, u):
+    """Solids with noslip and freeslip walls (records with the wall normal,
+    integrator/boundary.hpp:75-160), inflow/outflow/freeslip domain, AMR to
+    the geometry level."""
+    from tests.helpers import solid_case
+    case = solid_case(kind, model=model, base=4, top=6, geom=6, u=u)
+    tree = scene.build_tree(case)
+    assert len(tree.records) > 0
+    _run_pair(case, 5, f"{model}2d solid {kind} u {u}", tree=tree)
+
+
+SPHERE3D_TOML = """
+[scene]
+name = "sphere"
+domain_size = [2.0, 1.0, 1.0]
+[run]
+print_interval = 1.0
+end_time = 1.0
+[integrator]
+min_refinement_level = 3
+max_refinement_level = 5
+refinement_radius = 1.5
+error_threshold = [0.01, 0.05]
+error_weights = [1.0, 0.0, 0.0, 0.0, 0.1]
+max_cfl = 0.4
+[model.reference_values]
+rho = 1.0
+p = 1.0
+vel = [1.5, 0.0, 0.0]
+[initial_condition]
+type = "uniform"
+[[geometry]]
+name = "ball"
+path = "ball.stl"
+scaling = [0.3, 0.3, 0.3]
+translation = [-0.3, 0.05, 0.0]
+[boundary_conditions.ball]
+type = "noslip"
+[boundary_conditions.X.neg]
+type = "inflow"
+[boundary_conditions.X.pos]
+type = "outflow"
+[boundary_conditions.Y]
+type = "periodic"
+[boundary_conditions.Z.neg]
+type = "freeslip"
+[boundary_conditions.Z.pos]
+type = "freeslip"
+"""
+
+
+def test_parity_solid_stl_sphere_3d(tmp_path):
+    """A sphere read from an STL file (an icosphere written here), 3D Euler
+    at Mach 1.3 around it, periodic in y."""
+    from tests.test_geometry import icosphere, write_stl
+    v, f = icosphere(3)
+    write_stl(tmp_path / "ball.stl", [v[t] for t in f])
+    case = loads_case(SPHERE3D_TOML, "euler", 3, base_dir=str(tmp_path))
+    tree = scene.build_tree(case)
+    assert len(tree.records) > 0
+    _run_pair(case, 3, "euler3d STL sphere", tree=tree)
