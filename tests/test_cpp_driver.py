@@ -228,3 +228,29 @@ def test_reference_examples_compile(tmp_path):
         r = subprocess.run(["g++", "-std=c++20", "-fsyntax-only", "-I", os.path.join(inc, "dacti"), "-I", inc,
                             os.path.join(ex, f)], capture_output=True, text=True, timeout=120)
         assert r.returncode == 0, f + r.stderr[-3000:]
+
+
+def test_aero_force_linear_pressure(tmp_path):
+    """AeroForce (driver/observer/aero_force.hpp:28-113) on a cylinder
+    outline in the pressure field p_ref + a x + b y: the samples sit 0.01
+    inside the outline (radius R - 0.01) with the outline's segment lengths
+    and outward normals, so Cd = q a pi R (R - 0.01) and
+    Cl = q b pi R (R - 0.01), q = 2 / (rho |u|^2), to the grid's
+    resolution."""
+    if shutil.which("g++") is None:
+        pytest.skip("no g++")
+    from tests.helpers import solid_case
+    case = solid_case("cylinder", u=0.5)
+    exe = tmp_path / "aero_mock"
+    inc = os.path.join(ROOT, "include")
+    subprocess.run(["g++", "-std=c++17", "-O2", "-Wall", "-Werror", "-I", os.path.join(inc, "dacti"), "-I", inc,
+                    os.path.join(ROOT, "tests", "cpp", "aero_mock.cpp"), "-o", str(exe)], check=True)
+    (tmp_path / "c.toml").write_text(case.toml_text)
+    a, b, R, q = 0.3, -0.7, 0.15, 2.0 / (1.0 * 0.5 ** 2)
+    out = subprocess.run([str(exe), str(tmp_path / "c.toml"), str(tmp_path), repr(a), repr(b), "4096"],
+                         capture_output=True, text=True, timeout=60, check=True).stdout.split()
+    cl, cd = float(out[0]), float(out[1])
+    ring = np.pi * R * (R - 0.01)
+    h = 2.0 / 4096
+    assert cd == pytest.approx(q * a * ring, abs=q * abs(a) * 2 * np.pi * R * h)
+    assert cl == pytest.approx(q * b * ring, abs=q * abs(b) * 2 * np.pi * R * h)

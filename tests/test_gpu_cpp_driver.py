@@ -144,3 +144,41 @@ def test_integrate_hook_and_counters(run_case_exe, tmp_path):
     for h in hooks:
         assert float(h[4]) == gdt / (1 << int(h[1]))
     assert int(steps[-1][13]) == len(order_active(o.export()))
+
+
+def test_default_driver_aero_force_on_a_solid(run_case_exe, tmp_path):
+    """euler<2> around a noslip cylinder through default_driver with
+    AeroForce: the CSV carries Cl and Cd, the frames the Cp and
+    ObjectSurface fields, nonzero exactly on the cells flagged as object
+    boundaries (the oracle's is_boundary)."""
+    from tests.helpers import solid_case
+    probe = solid_case("cylinder", u=0.5)
+    o = Oracle(probe, scene.build_tree(probe))
+    dt, _ = o.integrate()
+    del o
+    text = probe.toml_text.replace("end_time = 1.0", f"end_time = {dt * 2.5!r}").replace(
+        "print_interval = 1.0", f"print_interval = {dt * 0.5!r}").replace("max_cfl = 0.4", "max_cfl = 0.4\nmax_dt = 1.0")
+    (tmp_path / "cases").mkdir()
+    path = tmp_path / "cases" / "cyl.toml"
+    path.write_text(text)
+    print("[cpp-driver] euler2d cylinder, default_driver + AeroForce, 3 steps", flush=True)
+    r = subprocess.run([str(run_case_exe), str(path), "euler2", "driver"], capture_output=True, text=True, timeout=300)
+    print(r.stdout[-1500:], r.stderr[-1500:], flush=True)
+    assert r.returncode == 0
+    out = tmp_path / "out" / "solid"
+    csv = (out / "solid_observables.csv").read_text().splitlines()
+    assert csv[0] == "time,frame,step,n_cells,Cl,Cd" and len(csv) == 4
+    case = loads_case(text, "euler", 2)
+    o = Oracle(case, scene.build_tree(case))
+    for _ in range(3):
+        o.integrate()
+    e = o.export()
+    idx = order_active(e)
+    arrays, _, ncells = read_vtu(out / "solid_0003.vtu")
+    assert ncells == len(idx)
+    surf = arrays["Aerodynamic Forces/ObjectSurface"].ravel()
+    np.testing.assert_array_equal(surf, ((e["flags"][idx] & 64) != 0).astype(float))
+    cp = arrays["Aerodynamic Forces/Cp"].ravel()
+    p_ref, rho_ref, u2 = 1.0, 1.0, 0.25
+    np.testing.assert_array_equal(cp[surf == 1], (e["p"][idx][surf == 1, 3] - p_ref) / (0.5 * rho_ref * u2))
+    assert (cp[surf == 0] == 0).all() and surf.sum() > 10
