@@ -124,6 +124,7 @@ class Oracle:
         self._fn("face_cover", None, [C.c_void_p, C.c_void_p])
         self._fn("set_no_gc", None, [C.c_void_p, C.c_int])
         self._fn("gc_now", None, [C.c_void_p])
+        self._fn("export_bc", None, [C.c_void_p, C.c_void_p, C.c_void_p])
 
     def _fn(self, name, res, args):
         f = getattr(self.L, self.pfx + name)
@@ -162,6 +163,14 @@ class Oracle:
     @property
     def acc_violations(self):
         return self._acc_violations(self.h)
+
+    def export_bc(self):
+        """(bc codes [n, 2D], object record of each node [n])."""
+        n, D = self.n_nodes, self.case.dim
+        bc = np.zeros((n, 2 * D), np.uint8)
+        brec = np.zeros(n, np.int32)
+        self._export_bc(self.h, bc.ctypes.data, brec.ctypes.data)
+        return bc, brec
 
     def export(self):
         from_sizes = sizes(self.case.model, self.case.dim)

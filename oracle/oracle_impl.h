@@ -1955,6 +1955,15 @@ void OPFX(t_boundary_value)(const o_params *P, int face, const double *p, double
     o_bnd b = bnd_domain(P, face);
     boundary_value(P, &b, &c, pb);
 }
+/* the boundary codes (NFACE per node) and object records of the pool */
+void OPFX(export_bc)(void *h, uint8_t *bc, int32_t *brec) {
+    o_state *S = (o_state *) h;
+    for (int64_t i = 0; i < S->n; ++i) {
+        for (int f = 0; f < NFACE; ++f) bc[(size_t) i * NFACE + f] = S->tree[i].bc[f];
+        brec[i] = S->tree[i].brec;
+    }
+}
+
 /* ghost state of an object boundary record (type, wall normal, inflow state, fade-in) */
 void OPFX(t_boundary_value_rec)(const o_params *P, int type, const double *normal, const double *pin, double fade,
                                 const double *p, double t, double *pb) {

@@ -174,3 +174,30 @@ def test_closed_box_conserves_mass_and_energy():
     e = o.export()
     lv = np.nonzero((e["flags"] & 3) == 3)[0]
     assert np.abs(e["u"][lv, 1] / e["u"][lv, 0] - 0.4).max() > 1e-3
+
+
+def test_split_children_keep_the_wall():
+    """Remeshing next to a solid (impl/adaptive_refinement.hpp:184-189):
+    children inherit their parent's outward faces, with the parent's object
+    record, so after several adaptive steps every object face of an active
+    leaf still has the cylinder's record, its normal pointing out of the
+    cylinder."""
+    case = solid_case("cylinder", base=3, top=6, geom=5, emin=0.002, emax=0.01, u=0.6)
+    t = scene.build_tree(case)
+    o = Oracle(case, t)
+    n0 = o.n_nodes
+    for _ in range(4):
+        _, fb = o.integrate()
+        assert not fb
+    e = o.export()
+    bc, brec = o.export_bc()
+    assert o.n_nodes > n0  # the mesh was refined
+    act = (e["flags"] & 3) == 3
+    wall = act & (bc == _lib.BC_OBJECT).any(axis=1)
+    assert wall.sum() > 20
+    assert (brec[wall] >= 0).all() and (brec[wall] < len(t.records)).all()
+    r = e["center"][wall] - [-0.4, 0.0]
+    nrm = t.records[brec[wall]]["normal"][:, :2]
+    assert np.min(np.sum(nrm * r / np.linalg.norm(r, axis=1)[:, None], axis=1)) > 0.99
+    # most wall cells were split past the geometry level during the run
+    assert (e["tree_level"][wall] == 6).sum() > (e["tree_level"][wall] == 5).sum()
