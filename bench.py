@@ -223,6 +223,102 @@ def learned(args):
     print(json.dumps(out), flush=True)
 
 
+def learned3d(args):
+    """Learned refinement predictor on 3D shock-drop runs (five-equation
+    stiffened gas, config 5): seeded Mach numbers uniform in [1.5, 3.5],
+    features (3x3x3 neighbourhood primitives + gradients), labels (the mesh
+    is finer at any of the next `horizon` global steps), inference and split
+    flags all computed in device memory; the BF16-MFMA MLP trains on the GPU;
+    a held-out Mach number is run remeshing in every substep (the reference),
+    with the estimator every 4 global steps, and every `every` steps with the
+    predictor refining ahead (estimator as fallback)."""
+    from dacti_amd import predictor, scene, workloads
+    from dacti_amd.devmem import DeviceArray
+    from dacti_amd.mlp import MLP
+
+    def log(msg):
+        print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+    h, every = args.horizon, args.every
+    rng = np.random.default_rng(args.seed + 500)
+    machs = [float(x) for x in rng.uniform(1.5, 3.5, args.train_runs + 1)]
+    Xs, ys, gs = [], [], []
+    t0 = time.perf_counter()
+    for k, mach in enumerate(machs[:-1]):
+        case = workloads.shock_drop(base=args.base, extra=args.extra, mach=mach, max_dt=args.drop_max_dt)
+        for X, y, gi in predictor.gpu_samples(case, args.train_steps, h):
+            Xs.append(X), ys.append(y), gs.append(gi)
+        log(f"training run {k} (Mach {mach:.2f}) done: {sum(len(y) for y in ys)} samples, "
+            f"{time.perf_counter() - t0:.0f} s")
+    X, y = np.concatenate(Xs), np.concatenate(ys)
+    del Xs, ys
+    std = predictor.Standardizer(X)
+    m = MLP(X.shape[1], 128, 3, 3, seed=args.seed)
+    loss = m.train(std(X), y, epochs=args.epochs, batch=1024, lr=2e-3, seed=1)
+    ms, fl = m.train_stats()
+    log(f"trained on {len(y)} samples x {X.shape[1]} features: loss {loss:.3f}, {fl / ms / 1e9:.2f} TFLOP/s")
+    case = workloads.shock_drop(base=args.base, extra=args.extra, mach=machs[-1], max_dt=args.drop_max_dt)
+    tree = scene.build_tree(case)
+    held = predictor.gpu_samples(case, args.train_steps, h, tree=tree)
+    Xt = np.concatenate([a for a, _, _ in held])
+    yt = np.concatenate([b for _, b, _ in held])
+    gt = np.concatenate([c for _, _, c in held])
+    pred = m.predict(std(Xt))
+    miss_net, miss_ind = predictor.missed_refinement(pred, yt), predictor.missed_refinement(gt, yt)
+    log(f"held-out Mach {machs[-1]:.2f}: missed refinement net {miss_net:.3f}, indicator {miss_ind:.3f}")
+    mu = DeviceArray.from_numpy(std.mu.astype(np.float32))
+    sd = DeviceArray.from_numpy(std.sd.astype(np.float32))
+    ref, t_ref, u_ref, _ = predictor.run_cadence_gpu(case, tree, args.steps, 1)
+    log(f"reference cadence: {t_ref:.2f} s")
+    p4, t_p4, u_p4, _ = predictor.run_cadence_gpu(case, tree, args.steps, 4)
+    log(f"estimator every 4 steps: {t_p4:.2f} s")
+    ln, t_ln, u_ln, flagged = predictor.run_cadence_gpu(case, tree, args.steps, every, model=m, mu=mu, sd=sd,
+                                                        rings=args.rings)
+    log(f"learned every {every} steps: {t_ln:.2f} s")
+    sweep = []
+    for spec in (args.cadences.split(",") if args.cadences else []):
+        ev, rg = (int(v) for v in spec.split(":"))
+        sn, st, su, sf = predictor.run_cadence_gpu(case, tree, args.steps, ev, model=m, mu=mu, sd=sd, rings=rg)
+        p_ev, t_ev, u_ev, _ = predictor.run_cadence_gpu(case, tree, args.steps, ev)
+        sweep.append({"every": ev, "rings": rg, "learned_seconds": st, "learned_cell_updates": su,
+                      "learned_rel_l1": predictor.relative_l1(ref, sn), "estimator_seconds": t_ev,
+                      "estimator_rel_l1": predictor.relative_l1(ref, p_ev), "flagged": sf})
+        log(f"sweep every {ev} rings {rg}: learned {st:.2f} s L1 {sweep[-1]['learned_rel_l1']:.4f}, "
+            f"estimator {t_ev:.2f} s L1 {sweep[-1]['estimator_rel_l1']:.4f}")
+    out = {
+        "metric": "cell-updates/s (HLLC+MUSCL FP64) & %HBM roofline; learned/batched vs plain",
+        "value": u_ln / t_ln,
+        "unit": "cell-updates/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": 0,
+        "ms_per_step": t_ln * 1e3 / args.steps,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"3D shock-drop (five-equation stiffened gas), octree levels {args.base}-"
+                               f"{args.base + args.extra}, learned refinement every {every} global steps vs the "
+                               f"estimator every 4 and every substep",
+                   "train_machs": machs[:-1], "held_out_mach": machs[-1], "train_samples": int(len(y)),
+                   "features": int(X.shape[1]), "mlp": "3x128 ReLU, BF16 MFMA, FP32 accumulation",
+                   "horizon_steps": h, "labels": "finer at any of the next horizon global steps",
+                   "dilation_rings": args.rings, "features_and_inference": "device memory",
+                   "max_dt": args.drop_max_dt},
+        "learned": {"cell_updates_per_s": u_ln / t_ln, "seconds": t_ln, "cell_updates": u_ln,
+                    "flagged_leaves": flagged, "rel_l1_vs_every_substep": predictor.relative_l1(ref, ln),
+                    "missed_refinement": miss_net, "accuracy": float((pred == yt).mean())},
+        "plain_every_4": {"cell_updates_per_s": u_p4 / t_p4, "seconds": t_p4, "cell_updates": u_p4,
+                          "rel_l1_vs_every_substep": predictor.relative_l1(ref, p4),
+                          "missed_refinement_of_present_flags": miss_ind},
+        "every_substep": {"cell_updates_per_s": u_ref / t_ref, "seconds": t_ref, "cell_updates": u_ref},
+        "training": {"loss": loss, "ms": ms, "gemm_tflops": fl / ms / 1e9},
+        "learned_speedup_vs_every_4": t_p4 / t_ln,
+        "learned_speedup_vs_every_substep": t_ref / t_ln,
+        "cadence_sweep": sweep,
+        "roofline": None,
+        "cpu_baseline": None,
+    }
+    m.close()
+    print(json.dumps(out), flush=True)
+
+
 def bubble(args):
     """2D shock-bubble (five-equation model): 64x32 base quadtree + `--extra`
     levels, Mach 1.22 shock in air hitting a helium-like bubble (gamma 1.67,
@@ -347,18 +443,23 @@ def main():
     ap.add_argument("--level", type=int, default=8)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-probe", action="store_true", help="run a 1 GiB device copy first (counter calibration)")
-    ap.add_argument("--workload", choices=("box", "batched", "learned", "drop", "bubble"), default="box",
+    ap.add_argument("--workload", choices=("box", "batched", "learned", "learned3d", "drop", "bubble"), default="box",
                     help="box: 3D periodic Euler box (default); batched: shock-bubble instances, batch vs plain; "
                          "learned: refinement predictor vs the estimator; drop: 3D shock-drop with AMR; "
                          "bubble: 2D shock-bubble with AMR")
     ap.add_argument("--graphs", action="store_true", help="bubble: persistent grid and graph replay")
     ap.add_argument("--base", type=int, default=5)
     ap.add_argument("--extra", type=int, default=4)
-    ap.add_argument("--horizon", type=int, default=16)
+    ap.add_argument("--horizon", type=int, default=None, help="learned: 16, learned3d: 8 global steps")
     ap.add_argument("--train-runs", type=int, default=3)
     ap.add_argument("--train-steps", type=int, default=48)
     ap.add_argument("--epochs", type=int, default=10)
     ap.add_argument("--rings", type=int, default=1, help="learned: face-neighbour rings added to predicted refinement")
+    ap.add_argument("--every", type=int, default=16, help="learned3d: remesh cadence of the predictor run")
+    ap.add_argument("--cadences", default="", help="learned3d: extra every:rings settings to compare, e.g. 4:1,2:2")
+    ap.add_argument("--drop-max-dt", type=float, default=1.6e-7,
+                    help="learned3d: global step cap (limits the LTS level spread; wider spreads break down at high "
+                         "Mach)")
     ap.add_argument("--instances", type=int, default=64)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--no-tune", action="store_true", help="batched: keep the default launch (no autotuning)")
@@ -376,6 +477,10 @@ def main():
     if args.workload == "batched":
         batched(args)
         return
+    if args.horizon is None:
+        args.horizon = 8 if args.workload == "learned3d" else 16
+    if args.workload == "learned3d":
+        return learned3d(args)
     if args.workload == "learned":
         learned(args)
         return

@@ -93,6 +93,11 @@ SIGNATURES = {
     "dacti_set_remesh": (C.c_int, [C.c_void_p, C.c_int32]),
     "dacti_mark_split": (C.c_int, [C.c_void_p, C.POINTER(C.c_int32), C.c_int64]),
     "dacti_destroy": (C.c_int, [C.c_void_p]),
+    "dacti_features": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                 C.c_void_p, C.c_int64, C.POINTER(C.c_int64), C.POINTER(C.c_int32)]),
+    "dacti_locate_levels": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p]),
+    "dacti_mark_predicted": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_int32,
+                                       C.POINTER(C.c_int64)]),
     "dacti_scene_build": (C.c_int, [C.c_char_p, C.c_char_p, C.c_int32, C.c_int32, C.c_int32, C.c_void_p, C.c_void_p,
                                     C.POINTER(C.c_void_p)]),
     "dacti_scene_case": (C.c_int, [C.c_void_p, C.POINTER(CaseStruct)]),

@@ -1962,6 +1962,150 @@ static __global__ void __launch_bounds__(256) k_pool_flush(const Item *__restric
 
 // flag listed leaves for splitting at the next remesh (the split keeps the
 // reference's conditions: an active leaf that may still split)
+// ------------------------------------------------- refinement predictor
+// Number of predictor features of a model: the leaf's primitives, the
+// differences to the primitives at the 3^D - 1 same-size neighbour positions,
+// the gradient (D x NG), the tree level above the base level and log10 of
+// the error estimate.
+template <int MODEL, int D>
+__host__ __device__ constexpr int n_features() {
+    using S = Sz<MODEL, D>;
+    return S::NE * (D == 2 ? 9 : 27) + D * S::NG + 2;
+}
+
+// the leaf containing point x (inside the domain): descent from the root
+// choosing each child by comparing x with the node's centre (a point on a
+// face goes to the negative side), so that ties do not depend on rounding
+// of scaled coordinates
+template <int D>
+__device__ inline int32_t leaf_at(const Dev &d, const double *x) {
+    int32_t idx = 1;
+    for (int guard = 0; guard < 64 && !(d.flags[idx] & F_LEAF); ++guard) {
+        int q = 0;
+        for (int a = 0; a < D; ++a)
+            if (x[a] > d.center[a * d.C + idx]) q |= 1 << a;
+        idx = d.child_base[idx] + q;
+    }
+    return idx;
+}
+
+static __global__ void k_active_flags(Dev d, int64_t n, uint8_t *__restrict__ fl) {
+    const int64_t i = blockIdx.x * (int64_t) blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint8_t f = d.flags[i];
+    fl[i] = (i > 0 && (f & F_LEAF) && (f & F_ACTIVE) && !(f & F_GARBAGE)) ? 1 : 0;
+}
+
+static __global__ void k_active_ids(int64_t n, const uint8_t *__restrict__ fl, const int32_t *__restrict__ off,
+                             int32_t *__restrict__ ids, int64_t cap) {
+    const int64_t i = blockIdx.x * (int64_t) blockDim.x + threadIdx.x;
+    if (i >= n || !fl[i] || off[i] >= cap) return;
+    ids[off[i]] = (int32_t) i;
+}
+
+// features of the listed leaves (one thread per leaf), standardised as
+// (x - mu) / sd in FP32 when mu is given.  A neighbour position is the
+// centre moved by one cell width; outside a non-periodic domain the leaf's
+// own state stands in (difference 0), a periodic axis wraps.
+template <int MODEL, int D>
+__global__ void k_features(Dev d, MConst c, const int32_t *__restrict__ ids, int64_t n, int min_level,
+                           const float *__restrict__ mu, const float *__restrict__ sd, float *__restrict__ out,
+                           double *__restrict__ centers, int8_t *__restrict__ levels) {
+    using S = Sz<MODEL, D>;
+    constexpr int NE = S::NE, NG = S::NG, NGX = S::NGX, NOFF = D == 2 ? 9 : 27, NFEAT = n_features<MODEL, D>();
+    const int64_t k = blockIdx.x * (int64_t) blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const int64_t C = d.C;
+    const int32_t y = ids[k];
+    float *o = out + k * NFEAT;
+    int col = 0;
+    auto put = [&](double v) {
+        float f = (float) v;
+        if (mu) f = (f - mu[col]) / sd[col];
+        o[col++] = f;
+    };
+    double pc[NE], ctr[D];
+    for (int e = 0; e < NE; ++e) pc[e] = d.p[e * C + y];
+    for (int a = 0; a < D; ++a) ctr[a] = d.center[a * C + y];
+    if (centers)
+        for (int a = 0; a < D; ++a) centers[k * D + a] = ctr[a];
+    if (levels) levels[k] = d.level[y];
+    const double sz = d.size[y];
+    for (int e = 0; e < NE; ++e) put(pc[e]);
+    for (int m = 0; m < NOFF; ++m) {
+        int off[D], r = m;
+        bool zero = true;
+        for (int a = D - 1; a >= 0; --a) {
+            off[a] = r % 3 - 1;
+            r /= 3;
+            zero = zero && off[a] == 0;
+        }
+        if (zero) continue;
+        double x[D];
+        bool inside = true;
+        for (int a = 0; a < D; ++a) {
+            x[a] = ctr[a] + sz * (double) off[a];
+            const double h = 0.5 * c.active_size[a];
+            if (!c.periodic[a]) inside = inside && fabs(x[a]) < h;
+            else if (x[a] < -h) x[a] += c.active_size[a];
+            else if (x[a] > h) x[a] -= c.active_size[a];
+        }
+        const int32_t nb = inside ? leaf_at<D>(d, x) : 0;
+        for (int e = 0; e < NE; ++e) put(nb > 0 ? d.p[e * C + nb] - pc[e] : 0.0);
+    }
+    for (int a = 0; a < D; ++a)
+        for (int e = 0; e < NG; ++e) put(d.grad[(a * NGX + e) * C + y]);
+    put((double) (d.level[y] - min_level));
+    put(log10(fmax(d.error[y], 1e-12)));
+}
+
+// tree level of the leaf containing each point (-1 outside the root cell)
+template <int D>
+__global__ void k_locate_levels(Dev d, MConst c, const double *__restrict__ pts, int64_t n, int8_t *__restrict__ lev) {
+    const int64_t k = blockIdx.x * (int64_t) blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    double x[D];
+    bool inside = true;
+    for (int a = 0; a < D; ++a) {
+        x[a] = pts[k * D + a];
+        inside = inside && fabs(x[a]) <= 0.5 * d.size[1];
+    }
+    lev[k] = inside ? d.level[leaf_at<D>(d, x)] : (int8_t) -1;
+}
+
+// leaves whose logits pick class 2 (refine): rn[y] = 1
+static __global__ void k_pred_seed(const float *__restrict__ logits, const int32_t *__restrict__ ids, int64_t n,
+                            uint8_t *__restrict__ rn) {
+    const int64_t k = blockIdx.x * (int64_t) blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const float *l = logits + 3 * k;
+    if (l[2] > l[0] && l[2] > l[1]) rn[ids[k]] = 1;
+}
+
+// one ring of face neighbours: active leaves next to a cell flagged r get r + 1
+template <int D>
+__global__ void k_pred_ring(Dev d, int64_t n, uint8_t *__restrict__ rn, uint8_t r) {
+    const int64_t i = blockIdx.x * (int64_t) blockDim.x + threadIdx.x;
+    if (i >= n || rn[i] != r) return;
+    for (int f = 0; f < 2 * D; ++f) {
+        const int32_t nb = d.nbr[f * d.C + i];
+        if (nb <= 0) continue;
+        const uint8_t fl = d.flags[nb];
+        if ((fl & F_LEAF) && (fl & F_ACTIVE) && !(fl & F_GARBAGE) && rn[nb] == 0) rn[nb] = (uint8_t) (r + 1);
+    }
+}
+
+// flag the leaves with rn > 0 to split at the next remesh (k_mark_split's rule)
+static __global__ void k_pred_mark(Dev d, int64_t n, const uint8_t *__restrict__ rn, unsigned long long *__restrict__ count) {
+    const int64_t y = blockIdx.x * (int64_t) blockDim.x + threadIdx.x;
+    if (y >= n || y < 1 || !rn[y]) return;
+    const uint8_t fl = d.flags[y];
+    if ((fl & F_LEAF) && (fl & F_ACTIVE) && !(fl & F_GARBAGE) && (fl & F_CAN_SPLIT)) {
+        d.flags[y] = fl | F_SHOULD_SPLIT;
+        atomicAdd(count, 1ull);
+    }
+}
+
 static __global__ void k_mark_split(Dev d, const int32_t *__restrict__ ids, int64_t n, int64_t n_nodes) {
     const int64_t i = blockIdx.x * (int64_t) blockDim.x + threadIdx.x;
     if (i >= n) return;

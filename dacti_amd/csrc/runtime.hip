@@ -833,6 +833,36 @@ int dacti_mark_split(dacti_handle *h, const int32_t *ids, int64_t n) {
     return DACTI_OK;
 }
 
+int dacti_features(dacti_handle *h, int32_t min_level, const float *mu, const float *sd, float *feat, int32_t *ids,
+                   double *centers, int8_t *levels, int64_t cap, int64_t *n, int32_t *n_features) {
+    if (!h || !n || !n_features || cap < 0 || (!mu) != (!sd)) return DACTI_E_ARG;
+    int rc = DACTI_E_ARG;
+#define DACTI_CASE_FEAT(M, Dm) \
+    if (h->model == M && h->D == Dm) rc = variant_features_##M##_##Dm(h, min_level, mu, sd, feat, ids, centers, levels, cap, n, \
+                                                          n_features);
+    DACTI_VARIANTS(DACTI_CASE_FEAT)
+    return rc;
+}
+
+int dacti_locate_levels(dacti_handle *h, const double *pts, int64_t n, int8_t *levels) {
+    if (!h || n < 0 || (n > 0 && (!pts || !levels))) return DACTI_E_ARG;
+    int rc = DACTI_E_ARG;
+#define DACTI_CASE_LOC(M, Dm) \
+    if (h->model == M && h->D == Dm) rc = variant_locate_##M##_##Dm(h, pts, n, levels);
+    DACTI_VARIANTS(DACTI_CASE_LOC)
+    return rc;
+}
+
+int dacti_mark_predicted(dacti_handle *h, const float *logits, const int32_t *ids, int64_t n, int32_t rings,
+                         int64_t *n_marked) {
+    if (!h || n < 0 || rings < 0 || !n_marked || (n > 0 && (!logits || !ids))) return DACTI_E_ARG;
+    int rc = DACTI_E_ARG;
+#define DACTI_CASE_MARK(M, Dm) \
+    if (h->model == M && h->D == Dm) rc = variant_mark_predicted_##M##_##Dm(h, logits, ids, n, rings, n_marked);
+    DACTI_VARIANTS(DACTI_CASE_MARK)
+    return rc;
+}
+
 int dacti_get_stats(dacti_handle *h, dacti_stats *s) {
     if (!h || !s) return DACTI_E_ARG;
     std::memset(s, 0, sizeof *s);

@@ -195,7 +195,13 @@ inline Item base_item(dacti_batch *b, int i) {
 #define DACTI_VARIANTS(X) X(EULER, 2) X(EULER, 3) X(NAVIER_STOKES, 2) X(NAVIER_STOKES, 3) X(MULTI, 2) X(MULTI, 3)
 #define DACTI_DECLARE_VARIANT(M, Dm) \
     int variant_init_##M##_##Dm(dacti_handle *h); \
-    int variant_integrate_##M##_##Dm(dacti_batch *b, int signal, int *force_break);
+    int variant_integrate_##M##_##Dm(dacti_batch *b, int signal, int *force_break); \
+    int variant_features_##M##_##Dm(dacti_handle *h, int min_level, const float *mu, const float *sd, float *feat, \
+                                    int32_t *ids, double *centers, int8_t *levels, int64_t cap, int64_t *n_out, \
+                                    int32_t *n_feat); \
+    int variant_locate_##M##_##Dm(dacti_handle *h, const double *pts, int64_t n, int8_t *lev); \
+    int variant_mark_predicted_##M##_##Dm(dacti_handle *h, const float *logits, const int32_t *ids, int64_t n, \
+                                          int rings, int64_t *n_marked);
 DACTI_VARIANTS(DACTI_DECLARE_VARIANT)
 
 template <int MODEL, int D>
@@ -275,6 +281,55 @@ struct Ops {
             HIPCHK(h, hipGetLastError());
             HIPCHK(h, hipStreamSynchronize(h->stream));
         }
+        return DACTI_OK;
+    }
+
+    // predictor features of the active leaves (ids ascending): k_features
+    static int features(dacti_handle *h, int min_level, const float *mu, const float *sd, float *feat, int32_t *ids,
+                        double *centers, int8_t *levels, int64_t cap, int64_t *n_out, int32_t *n_feat) {
+        *n_feat = n_features<MODEL, D>();
+        const int64_t n = h->n;
+        k_active_flags<<<blocks(n), 256, 0, h->stream>>>(h->d, n, h->d_rn);
+        size_t bytes = h->tmp_bytes;
+        HIPCHK(h, hipcub::DeviceScan::ExclusiveSum(h->d_tmp, bytes, h->d_rn, h->d_off, (int) n, h->stream));
+        int32_t last_off = 0;
+        uint8_t last_fl = 0;
+        HIPCHK(h, hipMemcpyAsync(&last_off, h->d_off + n - 1, 4, hipMemcpyDeviceToHost, h->stream));
+        HIPCHK(h, hipMemcpyAsync(&last_fl, h->d_rn + n - 1, 1, hipMemcpyDeviceToHost, h->stream));
+        HIPCHK(h, hipStreamSynchronize(h->stream));
+        *n_out = (int64_t) last_off + last_fl;
+        if (*n_out > cap || !feat || !ids) return DACTI_OK;  // sizes only
+        k_active_ids<<<blocks(n), 256, 0, h->stream>>>(n, h->d_rn, h->d_off, ids, cap);
+        if (*n_out > 0)
+            k_features<MODEL, D><<<blocks(*n_out, 128), 128, 0, h->stream>>>(h->d, h->mc, ids, *n_out, min_level, mu, sd,
+                                                                            feat, centers, levels);
+        HIPCHK(h, hipGetLastError());
+        HIPCHK(h, hipStreamSynchronize(h->stream));
+        return DACTI_OK;
+    }
+
+    static int locate(dacti_handle *h, const double *pts, int64_t n, int8_t *lev) {
+        if (n > 0) k_locate_levels<D><<<blocks(n), 256, 0, h->stream>>>(h->d, h->mc, pts, n, lev);
+        HIPCHK(h, hipGetLastError());
+        HIPCHK(h, hipStreamSynchronize(h->stream));
+        return DACTI_OK;
+    }
+
+    static int mark_predicted(dacti_handle *h, const float *logits, const int32_t *ids, int64_t n, int rings,
+                              int64_t *n_marked) {
+        const int64_t nn = h->n;
+        HIPCHK(h, hipMemsetAsync(h->d_rn, 0, (size_t) nn, h->stream));
+        if (n > 0) k_pred_seed<<<blocks(n), 256, 0, h->stream>>>(logits, ids, n, h->d_rn);
+        for (int r = 1; r <= rings && r < 255; ++r)
+            k_pred_ring<D><<<blocks(nn), 256, 0, h->stream>>>(h->d, nn, h->d_rn, (uint8_t) r);
+        unsigned long long *cnt = (unsigned long long *) h->d_hist;
+        HIPCHK(h, hipMemsetAsync(cnt, 0, sizeof(unsigned long long), h->stream));
+        k_pred_mark<<<blocks(nn), 256, 0, h->stream>>>(h->d, nn, h->d_rn, cnt);
+        unsigned long long c = 0;
+        HIPCHK(h, hipGetLastError());
+        HIPCHK(h, hipMemcpyAsync(&c, cnt, sizeof c, hipMemcpyDeviceToHost, h->stream));
+        HIPCHK(h, hipStreamSynchronize(h->stream));
+        *n_marked = (int64_t) c;
         return DACTI_OK;
     }
 };

@@ -259,13 +259,17 @@ type = "zeroGradient"
 """
 
 
-def shock_drop(base=5, extra=5, mach=2.4, radius=0.001, drop_x=0.0, shock_x=-0.003, error_threshold=(0.02, 0.05)):
+def shock_drop(base=5, extra=5, mach=2.4, radius=0.001, drop_x=0.0, shock_x=-0.003, error_threshold=(0.02, 0.05),
+               max_dt=None):
     """3D shock-drop, five-equation stiffened gas in SI units: a Mach `mach`
     shock in air (gamma 1.4, 1.2 kg/m^3, 1e5 Pa) hits a water drop (gamma
     6.12, P0 3.43e8 Pa, 1000 kg/m^3) in a 16 x 8 x 8 mm box (fluid 1 = water,
-    alpha = 1 inside the drop)."""
+    alpha = 1 inside the drop).  max_dt (integrator.max_dt) caps the global
+    step, and with it the spread of LTS levels."""
     r2, u2, p2 = normal_shock(1.2, 1.0e5, mach)
     toml = SHOCK_DROP_TOML.format(base=base, top=base + extra, emin=repr(float(error_threshold[0])),
                                   emax=repr(float(error_threshold[1])), shock_x=repr(float(shock_x)), r2=repr(r2),
                                   u2=repr(u2), p2=repr(p2), radius=repr(float(radius)), dx=repr(float(drop_x)))
+    if max_dt is not None:
+        toml = toml.replace("max_cfl = 0.5\n", f"max_cfl = 0.5\nmax_dt = {float(max_dt)!r}\n", 1)
     return loads_case(toml, "multi", 3)

@@ -164,6 +164,34 @@ int dacti_scene_case(const dacti_scene *s, dacti_case *out);
 int dacti_scene_tree(const dacti_scene *s, dacti_tree *out);
 int dacti_scene_destroy(dacti_scene *s);
 
+/*
+ * Refinement predictor on the GPU (an addition, no reference counterpart):
+ * dacti_features writes, for the active leaves of h (ids ascending), their
+ * feature rows: the primitives, the differences to the primitives at the
+ * 3^dim - 1 same-size neighbour positions (the leaf containing the centre
+ * moved by one cell width, found by descending from the root and comparing
+ * with node centres; zero outside a non-periodic domain), the gradient
+ * (dim x num_grad), the tree level minus min_level and log10 of the error
+ * estimate; *n_features = num_eqns * 3^dim + dim * num_grad + 2.  With mu
+ * and sd (device arrays of n_features floats) each value is standardised as
+ * (x - mu) / sd in FP32.  feat (cap x n_features floats), ids (cap) and
+ * the optional centers (cap x dim doubles) and levels (cap) of the leaves
+ * are device arrays; *n receives the number of active leaves, and nothing is
+ * written when it exceeds cap (call with cap 0 for the sizes).
+ * dacti_locate_levels gives the tree level of the leaf containing each of n
+ * points (device arrays: pts n x dim doubles, levels n; -1 outside the
+ * root cell), by the same descent.
+ * dacti_mark_predicted flags to split, at the next remesh, the listed
+ * leaves whose three logits (n x 3 device floats: coarsen, keep, refine)
+ * pick refine, plus `rings` rings of active face neighbours; *n_marked
+ * counts the flagged leaves.
+ */
+int dacti_features(dacti_handle *h, int32_t min_level, const float *mu, const float *sd, float *feat, int32_t *ids,
+                   double *centers, int8_t *levels, int64_t cap, int64_t *n, int32_t *n_features);
+int dacti_locate_levels(dacti_handle *h, const double *pts, int64_t n, int8_t *levels);
+int dacti_mark_predicted(dacti_handle *h, const float *logits, const int32_t *ids, int64_t n, int32_t rings,
+                         int64_t *n_marked);
+
 /* counters and timers of a handle */
 typedef struct dacti_stats {
     int64_t n_nodes, n_active;
