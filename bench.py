@@ -448,6 +448,8 @@ def main():
                          "learned: refinement predictor vs the estimator; drop: 3D shock-drop with AMR; "
                          "bubble: 2D shock-bubble with AMR")
     ap.add_argument("--graphs", action="store_true", help="bubble: persistent grid and graph replay")
+    ap.add_argument("--dense", type=int, default=1, choices=(0, 1, 2),
+                    help="box: flux op by windows for dense clusters (0 never, 1 auto, 2 always)")
     ap.add_argument("--base", type=int, default=5)
     ap.add_argument("--extra", type=int, default=4)
     ap.add_argument("--horizon", type=int, default=None, help="learned: 16, learned3d: 8 global steps")
@@ -501,6 +503,7 @@ def main():
     tree = scene.build_tree(case)
     print(f"[bench] tree built: {tree.n} nodes in {time.perf_counter() - t_build:.1f} s", file=sys.stderr, flush=True)
     g = Integrator(case, tree)
+    g.set_dense(args.dense)
     print("[bench] integrator created", file=sys.stderr, flush=True)
     for w in range(args.warmup):
         g.integrate()

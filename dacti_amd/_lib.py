@@ -91,6 +91,7 @@ SIGNATURES = {
     "dacti_set_timing": (C.c_int, [C.c_void_p, C.c_int32]),
     "dacti_set_launch": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32]),
     "dacti_set_remesh": (C.c_int, [C.c_void_p, C.c_int32]),
+    "dacti_set_dense": (C.c_int, [C.c_void_p, C.c_int32]),
     "dacti_mark_split": (C.c_int, [C.c_void_p, C.POINTER(C.c_int32), C.c_int64]),
     "dacti_destroy": (C.c_int, [C.c_void_p]),
     "dacti_features": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,

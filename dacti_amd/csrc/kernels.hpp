@@ -806,8 +806,14 @@ __device__ __forceinline__ void update_body(const Dev &d, const MConst &c, const
     }
     double gold[S::NGX], gnew[S::NGX];
     if constexpr (S::NG > 0) grad_input<MODEL, D>(c, p, gold);
+    double rv;
+    if (pow2_recip(vol, rv)) {
 #pragma unroll
-    for (int e = 0; e < NF; ++e) du[e] = d.tot[e * C + y] / vol;
+        for (int e = 0; e < NF; ++e) du[e] = d.tot[e * C + y] * rv;
+    } else {
+#pragma unroll
+        for (int e = 0; e < NF; ++e) du[e] = d.tot[e * C + y] / vol;
+    }
 #pragma unroll
     for (int e = 0; e < NE; ++e) u[e] += du[e];
     if constexpr (MODEL == MULTI) {
@@ -1067,12 +1073,21 @@ __device__ __forceinline__ void sweep_body(const Dev &d, const MConst &c, const 
         const double xc = 0.0;
         sweep_side<MODEL, D, 0>(d, c, y, dim, pc, ctr, ys, yt, ylev, L, pass, pn, qn, tn, xn, hn, deferred);
         sweep_side<MODEL, D, 1>(d, c, y, dim, pc, ctr, ys, yt, ylev, L, pass, pp, qp, tp, xp, hp, deferred);
-        double sl[NE];
-        for (int e = 0; e < NE; ++e) {
-            const double sn = (pc[e] - pn[e]) / (xc - xn);
-            const double sp = (pp[e] - pc[e]) / (xp - xc);
-            sl[e] = limiter(c.limiter, sn, sp);
+        double sl[NE], sn[NE], sp[NE];
+        double rn, rp;
+        const double dn = xc - xn, dp = xp - xc;
+        if (pow2_recip(dn, rn) && pow2_recip(dp, rp)) {
+            for (int e = 0; e < NE; ++e) {
+                sn[e] = (pc[e] - pn[e]) * rn;
+                sp[e] = (pp[e] - pc[e]) * rp;
+            }
+        } else {
+            for (int e = 0; e < NE; ++e) {
+                sn[e] = (pc[e] - pn[e]) / dn;
+                sp[e] = (pp[e] - pc[e]) / dp;
+            }
         }
+        for (int e = 0; e < NE; ++e) sl[e] = limiter(c.limiter, sn[e], sp[e]);
         if (pass == 0) {
             // rhs[i] gains the dim-j terms in the reference's (i, j, k) order
             rhs_add_dim<MODEL, D>(c, pc, sl, dim, kap, a_ip_in, a_in_ip, a_in_ia, rhs);
@@ -1081,8 +1096,15 @@ __device__ __forceinline__ void sweep_body(const Dev &d, const MConst &c, const 
                 double gp[S::NG], gn[S::NG];
                 grad_input<MODEL, D>(c, pp, gp);
                 grad_input<MODEL, D>(c, pn, gn);
-                for (int e = 0; e < S::NG; ++e)
-                    d.grad[(dim * S::NGX + e) * C + y] = (gp[e] - gn[e] - ts[e] * (tp - tn)) / (xp - xn);
+                double rg;
+                const double qg = xp - xn;
+                if (pow2_recip(qg, rg)) {
+                    for (int e = 0; e < S::NG; ++e)
+                        d.grad[(dim * S::NGX + e) * C + y] = (gp[e] - gn[e] - ts[e] * (tp - tn)) * rg;
+                } else {
+                    for (int e = 0; e < S::NG; ++e)
+                        d.grad[(dim * S::NGX + e) * C + y] = (gp[e] - gn[e] - ts[e] * (tp - tn)) / qg;
+                }
             }
         }
         // weighted_laplacian::estimate, error_estimators.hpp:19-37
@@ -1721,7 +1743,8 @@ struct Item {
     uint8_t *rn;
     int32_t *off;
     int64_t n_before;
-    int32_t meshing, pad2;
+    int32_t meshing;
+    int32_t walign;  // dense clusters: node ids of window starts are walign mod 64
 };
 constexpr int POOL_TILE = 2048;
 
@@ -1773,6 +1796,245 @@ struct OpDepositCoarse {
         deposit_coarse<MODEL, D>(it.d, it.rest[r], it.L);
     }
 };
+
+// ------------------------------------------------------- dense clusters
+// The lean flux op for a cluster that fills most of the pool (a uniform or
+// nearly uniform level), as one wavefront per aligned window of 64
+// consecutive node ids: a 4x4x4 (8x8) Morton block of a breadth-first
+// level.  A face between two same-level leaves of the window that both
+// update in this substep is evaluated once, by its left cell, and the
+// right cell takes the flux from LDS; every other face a cell needs is
+// evaluated in one compacted pass over the window.  Which side evaluates a
+// face, the fluxes received from finer neighbours, the deposits for
+// neighbours on other LTS levels and the per-cell face order of the sum
+// are flux_simple_body's (impl/flux.hpp:32-91), so the results equal that
+// op's bit for bit.  Cells with a face to another tree level go to the
+// rest list, as there.  One instance (items[0]); the windows cover the ids
+// from the cluster list's first to its last entry, aligned to it.walign.
+enum : int { WK_NONE = 0, WK_BOUNDARY = 1, WK_OWN = 2, WK_NEIGHBOUR = 3, WK_SHARED = 4 };
+
+template <int MODEL, int D>
+// three waves per SIMD (12.8 KB of LDS per wave allows no more): a few
+// spilled registers cost less than the latency hiding of the third wave
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) k_flux_window(const Item *__restrict__ items, const MConst *__restrict__ mcs) {
+    using S = Sz<MODEL, D>;
+    constexpr int NF = S::NF, NE = S::NE;
+    __shared__ double X[NF][64];     // + face flux of the current axis, evaluated by the lane's cell
+    __shared__ double M[D][NF][64];  // - face flux evaluated in the compacted pass
+    __shared__ int32_t PN[D][64];    // + neighbour of the lane's cell
+    __shared__ int32_t NM[D][64];    // - neighbour of the lane's cell
+    __shared__ int32_t CP[64];       // bit dim: the lane's cell evaluates its + face
+    __shared__ int32_t Q[D * 64];    // compacted pass: lane | dim << 6 | kind << 8 | deposit << 11
+    const Item &it = items[0];
+    const MConst &c = mcs[0];
+    const int nl = it.nl;
+    if (nl == 0) return;
+    const Dev &d = it.d;
+    const int64_t C = d.C;
+    const int L = it.L;
+    const double dt = it.dt;
+    const int64_t al = it.walign;
+    const int64_t w0 = ((int64_t) it.list[0] - al) >> 6, w1 = ((int64_t) it.list[nl - 1] - al) >> 6;
+    const int nw = (int) (w1 - w0 + 1);
+    const int lane = (int) threadIdx.x;
+    const uint64_t lt = lane ? (~0ull >> (64 - lane)) : 0ull;
+    for (int k = (int) blockIdx.x; k < nw; k += (int) gridDim.x) {
+        const int64_t base = al + (w0 + xcd_block(k, nw)) * 64;
+        const int64_t yid = base + lane;
+        const bool member = yid >= 1 && yid < it.n && d.cluster[yid] == L;
+        const int32_t y = (int32_t) yid;
+        int32_t nbs[2 * D];
+        int nac[2 * D], ncls[2 * D];
+        bool part = false;
+        int ylev = 0;
+#pragma unroll
+        for (int f = 0; f < 2 * D; ++f) nbs[f] = 0, nac[f] = 0, ncls[f] = -1;
+        if (member) {
+            ylev = d.level[y];
+            bool simple = true;
+#pragma unroll
+            for (int f = 0; f < 2 * D; ++f) {
+                const int32_t nb = d.nbr[f * C + y];
+                nbs[f] = nb;
+                nac[f] = nb ? d.acti[nb] : 0;
+                ncls[f] = nb ? d.cluster[nb] : -1;
+                if (nb != 0 && (!(d.flags[nb] & F_LEAF) || d.level[nb] != ylev)) simple = false;
+            }
+            if (!simple) it.rest[atomicAdd(&it.st->n_rest, 1)] = y;
+            part = simple;
+        }
+        const uint64_t pm = __ballot(part);
+        // a + face is evaluated by the lane's cell unless the neighbour is on
+        // a finer LTS level (whose deposit is received in the sum)
+        int cp = 0;
+#pragma unroll
+        for (int dim = 0; dim < D; ++dim) {
+            PN[dim][lane] = nbs[2 * dim + 1];
+            NM[dim][lane] = nbs[2 * dim];
+            if (part && (nbs[2 * dim + 1] == 0 || nac[2 * dim + 1] <= L)) cp |= 1 << dim;
+        }
+        CP[lane] = cp;
+        __syncthreads();
+        // - faces: the left cell's evaluation when it is a window lane that
+        // evaluates the face, else queued for the compacted pass
+        int kinds = 0;  // 3 bits per axis
+        int qn = 0;
+#pragma unroll
+        for (int dim = 0; dim < D; ++dim) {
+            int kind = WK_NONE, dep = 0;
+            if (part) {
+                const int f = 2 * dim;
+                const int32_t nb = nbs[f];
+                if (nb == 0) {
+                    kind = WK_BOUNDARY;
+                } else if (nac[f] < L) {
+                    kind = WK_OWN;
+                    dep = ncls[f] != L;
+                } else if (ncls[f] == L) {
+                    const int64_t o = (int64_t) nb - base;
+                    const bool inwin = o >= 0 && o < 64 && ((pm >> o) & 1);
+                    const int32_t nxt = inwin ? PN[dim][o] : d.nbr[(2 * dim + 1) * C + nb];
+                    if (nxt == y) kind = inwin && ((CP[o] >> dim) & 1) ? WK_SHARED : WK_NEIGHBOUR;
+                }
+            }
+            kinds |= kind << (3 * dim);
+            const bool q = kind == WK_BOUNDARY || kind == WK_OWN || kind == WK_NEIGHBOUR;
+            const uint64_t qm = __ballot(q);
+            if (q) Q[qn + __popcll(qm & lt)] = lane | dim << 6 | kind << 8 | dep << 11;
+            qn += __popcll(qm);
+        }
+        __syncthreads();
+        for (int j = lane; j < qn; j += 64) {
+            const int q = Q[j];
+            const int zl = q & 63, dim = (q >> 6) & 3, kind = (q >> 8) & 7;
+            const int32_t z = (int32_t) (base + zl);
+            double F[NF];
+            if (kind == WK_BOUNDARY) {
+                boundary_flux<MODEL, D>(d, c, z, 2 * dim, dim, 0, dt, F);
+            } else {
+                const int32_t nb = NM[dim][zl];
+                Side<MODEL, D> sz, sn;
+                load_side<MODEL, D>(d, z, dim, sz);
+                load_side<MODEL, D>(d, nb, dim, sn);
+                const double zc = d.center[dim * C + z], nc = d.center[dim * C + nb], zs = d.size[z];
+                const int zlev = d.level[z];
+                if (kind == WK_OWN) {
+                    face_flux_sides_body<MODEL, D>(c, sz, zc, zs, zlev, sn, nc, dim, 0, dt, F);
+                    if ((q >> 11) & 1) {
+#pragma unroll
+                        for (int e = 0; e < NF; ++e) {
+                            GMEM double &a = d.acc[((2 * dim + 1) * NF + e) * C + nb];
+                            a = a - F[e];
+                        }
+                    }
+                } else {
+                    face_flux_sides_body<MODEL, D>(c, sn, nc, zs, zlev, sz, zc, dim, 1, dt, F);
+                }
+            }
+#pragma unroll
+            for (int e = 0; e < NF; ++e) M[dim][e][zl] = F[e];
+        }
+        // + faces axis by axis; after each axis the cell adds its - and +
+        // face in its face order
+        double ys = 0.0, yc = 0.0;
+        Side<MODEL, D> sy;
+        double tot[NF];
+#pragma unroll
+        for (int e = 0; e < NF; ++e) tot[e] = 0.0;
+        if (part) {
+            ys = d.size[y];
+#pragma unroll
+            for (int e = 0; e < NE; ++e) {
+                sy.p[e] = d.p[e * C + y];
+                sy.r[e] = d.rhs[e * C + y];
+            }
+            sy.t = d.t[y];
+            if constexpr (MODEL == MULTI) sy.kap = d.k[S::IKAPPA * C + y];
+            if constexpr (S::NG > 0) {
+#pragma unroll
+                for (int a = 0; a < D; ++a)
+#pragma unroll
+                    for (int e = 0; e < S::NG; ++e) sy.g[a][e] = d.grad[(a * S::NGX + e) * C + y];
+            }
+        }
+#pragma unroll 1
+        for (int dim = 0; dim < D; ++dim) {
+            const int fm = 2 * dim, fp = 2 * dim + 1;
+            const int32_t nbp = pick_i<2 * D>(nbs, fp);
+            if ((cp >> dim) & 1) {
+#pragma unroll
+                for (int e = 0; e < NE; ++e) sy.s[e] = d.slope[(dim * NE + e) * C + y];
+                if constexpr (MODEL == MULTI) sy.kn = d.k[(S::IKN + dim) * C + y];
+                yc = d.center[dim * C + y];
+                double F[NF];
+                if (nbp == 0) {
+                    boundary_flux<MODEL, D>(d, c, y, fp, dim, 1, dt, F);
+                } else {
+                    Side<MODEL, D> sn;
+                    load_side<MODEL, D>(d, nbp, dim, sn);
+                    face_flux_sides_body<MODEL, D>(c, sy, yc, ys, ylev, sn, d.center[dim * C + nbp], dim, 1, dt, F);
+                    if (pick_i<2 * D>(ncls, fp) != L) {
+#pragma unroll
+                        for (int e = 0; e < NF; ++e) {
+                            GMEM double &a = d.acc[(fm * NF + e) * C + nbp];
+                            a = a + F[e];
+                        }
+                    }
+                }
+#pragma unroll
+                for (int e = 0; e < NF; ++e) X[e][lane] = F[e];
+            }
+            __syncthreads();
+            if (part) {
+                const int32_t nbm = pick_i<2 * D>(nbs, fm);
+                const int kind = (kinds >> (3 * dim)) & 7;
+                const bool takem = nbm != 0 && pick_i<2 * D>(nac, fm) > L;
+                const int o = (int) ((int64_t) nbm - base);
+                double cc[NF];
+#pragma unroll
+                for (int e = 0; e < NF; ++e) {
+                    cc[e] = 0.0;
+                    if (takem) {
+                        cc[e] = d.acc[(fm * NF + e) * C + y];
+                        d.acc[(fm * NF + e) * C + y] = 0.0;
+                    }
+                }
+                if (kind == WK_SHARED) {
+#pragma unroll
+                    for (int e = 0; e < NF; ++e) cc[e] = cc[e] + X[e][o];
+                } else if (kind == WK_BOUNDARY) {
+#pragma unroll
+                    for (int e = 0; e < NF; ++e) cc[e] = cc[e] - M[dim][e][lane];
+                } else if (kind != WK_NONE) {
+#pragma unroll
+                    for (int e = 0; e < NF; ++e) cc[e] = cc[e] + M[dim][e][lane];
+                }
+#pragma unroll
+                for (int e = 0; e < NF; ++e) tot[e] = tot[e] + cc[e];
+                const bool takep = nbp != 0 && pick_i<2 * D>(nac, fp) > L;
+#pragma unroll
+                for (int e = 0; e < NF; ++e) {
+                    cc[e] = 0.0;
+                    if (takep) {
+                        cc[e] = d.acc[(fp * NF + e) * C + y];
+                        d.acc[(fp * NF + e) * C + y] = 0.0;
+                    }
+                }
+                if ((cp >> dim) & 1) {
+#pragma unroll
+                    for (int e = 0; e < NF; ++e) cc[e] = cc[e] - X[e][lane];
+                }
+#pragma unroll
+                for (int e = 0; e < NF; ++e) tot[e] = tot[e] + cc[e];
+            }
+            __syncthreads();
+        }
+        if (part) {
+#pragma unroll
+            for (int e = 0; e < NF; ++e) d.tot[e * C + y] = tot[e];
+        }
+    }
+}
 
 template <int MODEL, int D>
 struct OpUpdate {

@@ -67,6 +67,20 @@ enum : uint8_t { BC_OBJECT = 7 };
 __device__ __forceinline__ double smin(double a, double b) { return (b < a) ? b : a; }  // std::min
 __device__ __forceinline__ double smax(double a, double b) { return (a < b) ? b : a; }  // std::max
 
+// Division by an exact power of two: x / 2^k and x * 2^-k are the same
+// correctly rounded value (also for zeros, infinities and subnormal
+// results), so where a divisor is a power of two -- every cell size and
+// volume of a domain whose root cell has a power-of-two side -- a product
+// with the exact reciprocal replaces the division without changing a bit.
+// Returns whether q is such a divisor (normal, with a normal reciprocal)
+// and sets r = 1 / q.
+__device__ __forceinline__ bool pow2_recip(double q, double &r) {
+    const unsigned long long b = (unsigned long long) __double_as_longlong(q);
+    const unsigned long long e = (b >> 52) & 0x7ffull;
+    r = __longlong_as_double((long long) ((b & 0x8000000000000000ull) | ((2046ull - e) << 52)));
+    return (b & 0xfffffffffffffull) == 0 && e >= 2 && e <= 2045;
+}
+
 // ---------------------------------------------------------------- EOS
 // multi.hpp:367-387
 __device__ inline double mgamma(const MConst &c, double a) {

@@ -243,7 +243,8 @@ int integrate_any(dacti_handle *h, int signal, int *fb) {
 void batch_drop_graphs(dacti_batch *b) {
     if (b->g_head) (void) hipGraphExecDestroy(b->g_head);
     if (b->g_tail) (void) hipGraphExecDestroy(b->g_tail);
-    b->g_head = b->g_tail = nullptr;
+    if (b->g_head_dense) (void) hipGraphExecDestroy(b->g_head_dense);
+    b->g_head = b->g_tail = b->g_head_dense = nullptr;
 }
 
 void batch_free(dacti_batch *b) {
@@ -443,6 +444,15 @@ int dacti_create(const dacti_case *c, const dacti_tree *t, dacti_handle **out) {
         return DACTI_E_HIP;
     }
     dacti_handle *h = new dacti_handle();
+    {
+        // windows of the dense flux op start where the deepest level of the
+        // initial tree starts (a breadth-first level is in Morton order)
+        int lmax = -1;
+        int64_t first = 1;
+        for (int64_t i = 1; i < t->n_nodes; ++i)
+            if (t->tree_level[i] > lmax) lmax = t->tree_level[i], first = i;
+        h->walign = (int) (first & 63);
+    }
     h->model = c->model;
     h->D = c->dim;
     h->sz = model_sizes(c->model, c->dim);
@@ -805,6 +815,16 @@ int dacti_set_substep_hook(dacti_handle *h, dacti_substep_hook fn, void *user) {
     if (!h) return DACTI_E_ARG;
     h->hook = fn;
     h->hook_user = user;
+    return DACTI_OK;
+}
+
+int dacti_set_dense(dacti_handle *h, int32_t mode) {
+    if (!h) return DACTI_E_ARG;
+    if (mode < 0 || mode > 2) {
+        h->err = "dacti_set_dense: mode must be 0, 1 or 2";
+        return DACTI_E_ARG;
+    }
+    h->dense = mode;
     return DACTI_OK;
 }
 

@@ -93,6 +93,8 @@ struct dacti_handle {
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     struct dacti_batch *solo = nullptr;    // batch of this handle alone
     int remesh = 1;  // 0: refinement marks paused (host-set cadence)
+    int dense = 1;   // window flux op for dense clusters: 0 never, 1 when the cluster fills half the pool, 2 always
+    int walign = 0;  // window alignment: first node id of the initial tree's deepest level, mod 64
     dacti_substep_hook hook = nullptr;  // called after every substep (impl/integrate.hpp:64)
     void *hook_user = nullptr;
     std::string err;
@@ -121,6 +123,8 @@ struct dacti_batch {
     int block = 256, grid = 0;
     bool graphs = false;
     hipGraphExec_t g_head = nullptr, g_tail = nullptr;
+    hipGraphExec_t g_head_dense = nullptr;  // head with the window flux op (a dense cluster of a batch of one)
+    bool dense = false;                     // this substep takes the window flux op
     bool timing = false;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     double flux_ms = 0, substep_ms = 0;
@@ -188,6 +192,7 @@ inline Item base_item(dacti_batch *b, int i) {
     it.hist = b->d_hist + 16 * i;
     it.n = h->n;
     it.remesh = h->remesh;
+    it.walign = h->walign;
     return it;
 }
 

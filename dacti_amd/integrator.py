@@ -125,6 +125,11 @@ class Integrator:
         """Launch configuration of this integrator's stepping (results do not depend on it)."""
         self._check(self.lib.dacti_set_launch(self.h, block, grid, int(bool(graphs))), "set_launch")
 
+    def set_dense(self, mode):
+        """Flux op for dense clusters: 0 never, 1 when a cluster fills half
+        the pool (default), 2 always (results do not depend on it)."""
+        self._check(self.lib.dacti_set_dense(self.h, int(mode)), "set_dense")
+
     def set_remesh(self, on):
         """Pause (False) or resume (True) refinement marks; an extension of
         the reference, which remeshes in every substep."""

@@ -261,6 +261,14 @@ int dacti_set_timing(dacti_handle *h, int32_t on);
  * on it; small adaptive cases gain from graph replay. */
 int dacti_set_launch(dacti_handle *h, int32_t block, int32_t grid, int32_t graphs);
 
+/* Flux op for dense clusters (no reference counterpart; results do not
+ * depend on it): a level's cluster that fills at least half of the node pool
+ * is stepped by windows of 64 consecutive node ids, each face between two
+ * updating same-level leaves of a window evaluated once.  mode 0 never, 1
+ * when the cluster is that dense (the default), 2 for every cluster of a
+ * handle stepped alone (batches always use the per-cell op). */
+int dacti_set_dense(dacti_handle *h, int32_t mode);
+
 /* Remeshing cadence (an extension; the reference remeshes in every substep,
  * impl/integrate.hpp:59): with on = 0 the sweep sets no split or unsplit
  * marks, so the mesh stays as it is while the LTS levels still adapt. */
