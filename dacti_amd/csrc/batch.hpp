@@ -48,7 +48,12 @@ struct BOps {
         k_batch<<<gl, B, 0, st>>>(it, b->d_mcs, b->d_toff, n, OpFluxRest<MODEL, D>{});
         k_batch<<<gl, B, 0, st>>>(it, b->d_mcs, b->d_toff, n, OpDepositCoarse<MODEL, D>{});
         k_batch<<<gl, B, 0, st>>>(it, b->d_mcs, b->d_toff, n, OpUpdate<MODEL, D>{});
-        k_batch<<<gl, B, 0, st>>>(it, b->d_mcs, b->d_toff, n, OpSweep<MODEL, D>{0});
+        if (b->dense) {
+            k_sweep_window<MODEL, D><<<window_grid(b), 64, 0, st>>>(it, b->d_mcs);
+            k_batch<<<gl, B, 0, st>>>(it, b->d_mcs, b->d_toff, n, OpSweepRest<MODEL, D>{});
+        } else {
+            k_batch<<<gl, B, 0, st>>>(it, b->d_mcs, b->d_toff, n, OpSweep<MODEL, D>{0});
+        }
         if constexpr (MODEL == MULTI) {
             for (int r = 0; r < 3; ++r) {
                 k_batch<<<gl, B, 0, st>>>(it, b->d_mcs, b->d_toff, n, OpSmoothAlpha<MODEL, D>{});

@@ -51,6 +51,7 @@ struct Status {
     int32_t n_unsplit;
     int32_t n_deferred;
     int32_t n_rest;         // cells left to the general flux kernel
+    int32_t n_rest2;        // cells of a dense cluster left to the per-cell sweep
     int32_t kept;
     int32_t lowered;
     int32_t dropped;
@@ -1042,6 +1043,105 @@ __device__ __forceinline__ void sweep_side(const Dev &d, const MConst &c, int32_
     xv = DIR ? 0.5 * (ys + ns) : -0.5 * (ys + ns);
 }
 
+// the end of a cell's sweep: error, split / unsplit marks
+// (impl/adaptive_refinement.hpp:18-94) or, for an unmarked cell, the LTS
+// decision (:264-322); a deferred cell only keeps its mark
+template <int MODEL, int D>
+__device__ __forceinline__ void sweep_decide(const Dev &d, const MConst &c, int32_t y, const double *pc, const double *ctr,
+                                             double ys, int ylev, int L, double dt, int counter, bool deferred, double err,
+                                             Status *st, int remesh) {
+    bool lowered = false, marked = false;
+    if (deferred) {
+        d.mark[y] = MK_DEFER;
+    } else {
+        err = sqrt(err);
+        d.error[y] = err;
+        uint8_t fl = d.flags[y];
+        if (fl & F_CAN_SPLIT) {
+            if (!scene_can_split<D>(c, ctr, ylev)) fl &= ~F_CAN_SPLIT;
+            d.flags[y] = fl;
+        }
+        uint8_t m = 0;
+        // with remeshing paused (a cadence set by the host, not a reference
+        // behaviour) every cell takes the LTS decision
+        if (remesh) {
+            if ((err > c.max_error && (fl & F_CAN_SPLIT)) || (fl & F_SHOULD_SPLIT)) m |= MK_SPLIT;
+            if (err < c.min_error && (fl & F_CAN_UNSPLIT) && ylev > 0 && counter == 1) m |= MK_UNSPLIT_CAND;
+        }
+        if (m == 0) {
+            lts_decide<MODEL, D>(d, c, y, pc, ys, L, dt, counter, lowered);
+            m = MK_RA_DONE;
+        } else {
+            marked = true;
+        }
+        d.mark[y] = m;
+    }
+    wave_count_add(deferred, &st->n_deferred);
+    wave_count_add(marked, &st->n_marked);
+    wave_count_add(lowered, &st->lowered);
+}
+
+// One axis of a cell's sweep from its two neighbour values (pn | pp: the
+// values the slope uses, qn | qp: those the error estimator uses, hn / hp:
+// false at a domain face): the limited slope (impl/slopes.hpp:15-76), the
+// rhs terms of this axis, the gradient and the Laplacian error term
+// (error_estimators.hpp:19-37).
+template <int MODEL, int D>
+__device__ __forceinline__ void sweep_axis(const Dev &d, const MConst &c, int32_t y, int dim, int pass, const double *pc,
+                                           const double *pn, const double *pp, const double *qn, const double *qp,
+                                           double xn, double xp, double tn, double tp, bool hn, bool hp,
+                                           const double *ts, double kap, double a_ip_in, double a_in_ip, double a_in_ia,
+                                           double *rhs, double &err) {
+    using S = Sz<MODEL, D>;
+    constexpr int NE = S::NE;
+    const int64_t C = d.C;
+    const double xc = 0.0;
+    double sl[NE], sn[NE], sp[NE];
+    double rn, rp;
+    const double dn = xc - xn, dp = xp - xc;
+    if (pow2_recip(dn, rn) && pow2_recip(dp, rp)) {
+        for (int e = 0; e < NE; ++e) {
+            sn[e] = (pc[e] - pn[e]) * rn;
+            sp[e] = (pp[e] - pc[e]) * rp;
+        }
+    } else {
+        for (int e = 0; e < NE; ++e) {
+            sn[e] = (pc[e] - pn[e]) / dn;
+            sp[e] = (pp[e] - pc[e]) / dp;
+        }
+    }
+    for (int e = 0; e < NE; ++e) sl[e] = limiter(c.limiter, sn[e], sp[e]);
+    if (pass == 0) {
+        // rhs[i] gains the dim-j terms in the reference's (i, j, k) order
+        rhs_add_dim<MODEL, D>(c, pc, sl, dim, kap, a_ip_in, a_in_ip, a_in_ia, rhs);
+        for (int e = 0; e < NE; ++e) __builtin_nontemporal_store(sl[e], &d.slope[(dim * NE + e) * C + y]);
+        if constexpr (S::NG > 0) {
+            double gp[S::NG], gn[S::NG];
+            grad_input<MODEL, D>(c, pp, gp);
+            grad_input<MODEL, D>(c, pn, gn);
+            double rg;
+            const double qg = xp - xn;
+            if (pow2_recip(qg, rg)) {
+                for (int e = 0; e < S::NG; ++e)
+                    d.grad[(dim * S::NGX + e) * C + y] = (gp[e] - gn[e] - ts[e] * (tp - tn)) * rg;
+            } else {
+                for (int e = 0; e < S::NG; ++e)
+                    d.grad[(dim * S::NGX + e) * C + y] = (gp[e] - gn[e] - ts[e] * (tp - tn)) / qg;
+            }
+        }
+    }
+    // weighted_laplacian::estimate, error_estimators.hpp:19-37
+    double sq = 0.0;
+    for (int e = 0; e < NE; ++e) {
+        double d2 = pc[e] * -2.0;
+        d2 = hn ? d2 + qn[e] : 0.0;
+        d2 = (hn && hp) ? d2 + qp[e] : 0.0;
+        const double v = d2 * c.weights[e];
+        sq += v * v;
+    }
+    err += sq;
+}
+
 template <int MODEL, int D>
 __device__ __forceinline__ void sweep_body(const Dev &d, const MConst &c, const int32_t *__restrict__ list, int nl, int li, int L,
                                                 double dt, int counter, int pass, Status *st, int remesh) {
@@ -1073,82 +1173,12 @@ __device__ __forceinline__ void sweep_body(const Dev &d, const MConst &c, const 
         const double xc = 0.0;
         sweep_side<MODEL, D, 0>(d, c, y, dim, pc, ctr, ys, yt, ylev, L, pass, pn, qn, tn, xn, hn, deferred);
         sweep_side<MODEL, D, 1>(d, c, y, dim, pc, ctr, ys, yt, ylev, L, pass, pp, qp, tp, xp, hp, deferred);
-        double sl[NE], sn[NE], sp[NE];
-        double rn, rp;
-        const double dn = xc - xn, dp = xp - xc;
-        if (pow2_recip(dn, rn) && pow2_recip(dp, rp)) {
-            for (int e = 0; e < NE; ++e) {
-                sn[e] = (pc[e] - pn[e]) * rn;
-                sp[e] = (pp[e] - pc[e]) * rp;
-            }
-        } else {
-            for (int e = 0; e < NE; ++e) {
-                sn[e] = (pc[e] - pn[e]) / dn;
-                sp[e] = (pp[e] - pc[e]) / dp;
-            }
-        }
-        for (int e = 0; e < NE; ++e) sl[e] = limiter(c.limiter, sn[e], sp[e]);
-        if (pass == 0) {
-            // rhs[i] gains the dim-j terms in the reference's (i, j, k) order
-            rhs_add_dim<MODEL, D>(c, pc, sl, dim, kap, a_ip_in, a_in_ip, a_in_ia, rhs);
-            for (int e = 0; e < NE; ++e) __builtin_nontemporal_store(sl[e], &d.slope[(dim * NE + e) * C + y]);
-            if constexpr (S::NG > 0) {
-                double gp[S::NG], gn[S::NG];
-                grad_input<MODEL, D>(c, pp, gp);
-                grad_input<MODEL, D>(c, pn, gn);
-                double rg;
-                const double qg = xp - xn;
-                if (pow2_recip(qg, rg)) {
-                    for (int e = 0; e < S::NG; ++e)
-                        d.grad[(dim * S::NGX + e) * C + y] = (gp[e] - gn[e] - ts[e] * (tp - tn)) * rg;
-                } else {
-                    for (int e = 0; e < S::NG; ++e)
-                        d.grad[(dim * S::NGX + e) * C + y] = (gp[e] - gn[e] - ts[e] * (tp - tn)) / qg;
-                }
-            }
-        }
-        // weighted_laplacian::estimate, error_estimators.hpp:19-37
-        double sq = 0.0;
-        for (int e = 0; e < NE; ++e) {
-            double d2 = pc[e] * -2.0;
-            d2 = hn ? d2 + qn[e] : 0.0;
-            d2 = (hn && hp) ? d2 + qp[e] : 0.0;
-            const double v = d2 * c.weights[e];
-            sq += v * v;
-        }
-        err += sq;
+        sweep_axis<MODEL, D>(d, c, y, dim, pass, pc, pn, pp, qn, qp, xn, xp, tn, tp, hn, hp, ts, kap, a_ip_in, a_in_ip,
+                             a_in_ia, rhs, err);
     }
     if (pass == 0)
         for (int e = 0; e < NE; ++e) __builtin_nontemporal_store(rhs[e], &d.rhs[e * C + y]);
-    bool lowered = false, marked = false;
-    if (deferred) {
-        d.mark[y] = MK_DEFER;
-    } else {
-        err = sqrt(err);
-        d.error[y] = err;
-        uint8_t fl = d.flags[y];
-        if (fl & F_CAN_SPLIT) {
-            if (!scene_can_split<D>(c, ctr, ylev)) fl &= ~F_CAN_SPLIT;
-            d.flags[y] = fl;
-        }
-        uint8_t m = 0;
-        // with remeshing paused (a cadence set by the host, not a reference
-        // behaviour) every cell takes the LTS decision
-        if (remesh) {
-            if ((err > c.max_error && (fl & F_CAN_SPLIT)) || (fl & F_SHOULD_SPLIT)) m |= MK_SPLIT;
-            if (err < c.min_error && (fl & F_CAN_UNSPLIT) && ylev > 0 && counter == 1) m |= MK_UNSPLIT_CAND;
-        }
-        if (m == 0) {
-            lts_decide<MODEL, D>(d, c, y, pc, ys, L, dt, counter, lowered);
-            m = MK_RA_DONE;
-        } else {
-            marked = true;
-        }
-        d.mark[y] = m;
-    }
-    wave_count_add(deferred, &st->n_deferred);
-    wave_count_add(marked, &st->n_marked);
-    wave_count_add(lowered, &st->lowered);
+    sweep_decide<MODEL, D>(d, c, y, pc, ctr, ys, ylev, L, dt, counter, deferred, err, st, remesh);
 }
 
 // apply the stored LTS decisions (impl/adaptive_refinement.hpp:308-321) and
@@ -2035,6 +2065,162 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) k_
         }
     }
 }
+
+// The sweep (slopes, rhs, error, marks, LTS decision) of a dense cluster by
+// the windows of k_flux_window.  A window stages the states of its 64 ids in
+// LDS and loads those of its members' out-of-window neighbours (the halo)
+// into LDS in one batch, so a cell's neighbour reads wait for two rounds of
+// memory latency instead of one per face.  Cells with a neighbour that is
+// not a same-level leaf go to the rest list (counted in n_rest2; the flux
+// ops are done with it) for the per-cell sweep.  Both evaluate sweep_axis
+// and sweep_decide on the same values, so the results are the same bit for
+// bit.
+template <int MODEL, int D>
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) k_sweep_window(const Item *__restrict__ items, const MConst *__restrict__ mcs) {
+    using S = Sz<MODEL, D>;
+    constexpr int NE = S::NE, HC = 128;
+    __shared__ double P[NE][64];   // states of the window's ids
+    __shared__ double T[64];       // their times (gradients only)
+    __shared__ double H[NE][HC];   // halo states
+    __shared__ double HT[HC];
+    __shared__ int32_t HN[HC];     // halo node ids
+    __shared__ int32_t FLV[64];    // flags | level << 8 of the window's ids
+    __shared__ int32_t HFLV[HC];
+    const Item &it = items[0];
+    const MConst &c = mcs[0];
+    const int nl = it.nl;
+    if (nl == 0 || it.st->nan_flag) return;  // integrate() stops after the update (impl/integrate.hpp:50)
+    const Dev &d = it.d;
+    const int64_t C = d.C;
+    const int L = it.L;
+    const int64_t al = it.walign;
+    const int64_t w0 = ((int64_t) it.list[0] - al) >> 6, w1 = ((int64_t) it.list[nl - 1] - al) >> 6;
+    const int nw = (int) (w1 - w0 + 1);
+    const int lane = (int) threadIdx.x;
+    const uint64_t lt = lane ? (~0ull >> (64 - lane)) : 0ull;
+    for (int k = (int) blockIdx.x; k < nw; k += (int) gridDim.x) {
+        const int64_t base = al + (w0 + xcd_block(k, nw)) * 64;
+        const int64_t yid = base + lane;
+        const bool valid = yid >= 1 && yid < it.n;
+        const int32_t y = (int32_t) yid;
+        bool member = false;
+        int flv = 0;
+        if (valid) {
+            member = d.cluster[y] == L;
+            flv = d.flags[y] | ((uint8_t) d.level[y]) << 8;
+#pragma unroll
+            for (int e = 0; e < NE; ++e) P[e][lane] = d.p[e * C + y];
+            if constexpr (S::NG > 0) T[lane] = d.t[y];
+        }
+        FLV[lane] = flv;
+        int32_t nbs[2 * D];
+        int hs[2 * D];
+        int qn = 0;
+#pragma unroll
+        for (int f = 0; f < 2 * D; ++f) {
+            const int32_t nb = member ? d.nbr[f * C + y] : 0;
+            nbs[f] = nb;
+            const int64_t o = (int64_t) nb - base;
+            const bool out = nb != 0 && !(o >= 0 && o < 64);
+            const uint64_t m = __ballot(out);
+            const int pos = qn + __popcll(m & lt);
+            hs[f] = out ? pos : -1;
+            if (out && pos < HC) HN[pos] = nb;
+            qn += __popcll(m);
+        }
+        __syncthreads();
+        const int qh = qn < HC ? qn : HC;
+        for (int j = lane; j < qh; j += 64) {
+            const int32_t nb = HN[j];
+#pragma unroll
+            for (int e = 0; e < NE; ++e) H[e][j] = d.p[e * C + nb];
+            if constexpr (S::NG > 0) HT[j] = d.t[nb];
+            HFLV[j] = d.flags[nb] | ((uint8_t) d.level[nb]) << 8;
+        }
+        __syncthreads();
+        if (member) {
+            const int ylev = (int8_t) (flv >> 8);
+            bool fast = true;
+#pragma unroll
+            for (int f = 0; f < 2 * D; ++f) {
+                const int32_t nb = nbs[f];
+                if (nb == 0) continue;
+                const int h = hs[f];
+                const int v = h < 0 ? FLV[nb - base] : h < HC ? HFLV[h] : (d.flags[nb] | ((uint8_t) d.level[nb]) << 8);
+                if (!(v & F_LEAF) || ((v >> 8) & 0xff) != (flv >> 8)) fast = false;
+            }
+            if (!fast) {
+                it.rest[atomicAdd(&it.st->n_rest2, 1)] = y;
+            } else {
+                double pc[NE], ctr[D], rhs[NE];
+#pragma unroll
+                for (int e = 0; e < NE; ++e) pc[e] = P[e][lane];
+#pragma unroll
+                for (int a = 0; a < D; ++a) ctr[a] = d.center[a * C + y];
+                const double ys = d.size[y], yt = d.t[y];
+                const double kap = MODEL == MULTI ? d.k[S::IKAPPA * C + y] : 0.0;
+                double a_ip_in, a_in_ip, a_in_ia;
+                rhs_coeffs<MODEL, D>(c, pc, kap, a_ip_in, a_in_ip, a_in_ia);
+#pragma unroll
+                for (int e = 0; e < NE; ++e) rhs[e] = 0.0;
+                double ts[S::NGX];
+                if constexpr (S::NG > 0)
+                    for (int e = 0; e < S::NG; ++e) ts[e] = d.t_slope[e * C + y];
+                // a neighbour's state: a domain face's ghost, the window, the
+                // halo, or (beyond the halo's capacity) global memory
+                auto fetch = [&](int f, int dir, double *pv, double &tv, double &xv, bool &hv) {
+                    const int32_t nb = pick_i<2 * D>(nbs, f);
+                    if (nb == 0) {
+                        boundary_value_at<MODEL, D>(d, c, y, f, pc, yt, pv);
+                        xv = dir ? ys : -ys;
+                        tv = yt;
+                        hv = false;
+                        return;
+                    }
+                    hv = true;
+                    const int h = pick_i<2 * D>(hs, f);
+                    tv = 0.0;
+                    if (h < 0) {
+                        const int o = (int) (nb - base);
+#pragma unroll
+                        for (int e = 0; e < NE; ++e) pv[e] = P[e][o];
+                        if constexpr (S::NG > 0) tv = T[o];
+                    } else if (h < HC) {
+#pragma unroll
+                        for (int e = 0; e < NE; ++e) pv[e] = H[e][h];
+                        if constexpr (S::NG > 0) tv = HT[h];
+                    } else {
+#pragma unroll
+                        for (int e = 0; e < NE; ++e) pv[e] = d.p[e * C + nb];
+                        tv = d.t[nb];
+                    }
+                    xv = dir ? 0.5 * (ys + ys) : -0.5 * (ys + ys);
+                };
+                double err = 0.0;
+#pragma unroll 1
+                for (int dim = 0; dim < D; ++dim) {
+                    double pn[NE], pp[NE], xn, xp, tn, tp;
+                    bool hn, hp;
+                    fetch(2 * dim, 0, pn, tn, xn, hn);
+                    fetch(2 * dim + 1, 1, pp, tp, xp, hp);
+                    sweep_axis<MODEL, D>(d, c, y, dim, 0, pc, pn, pp, pn, pp, xn, xp, tn, tp, hn, hp, ts, kap, a_ip_in,
+                                         a_in_ip, a_in_ia, rhs, err);
+                }
+#pragma unroll
+                for (int e = 0; e < NE; ++e) __builtin_nontemporal_store(rhs[e], &d.rhs[e * C + y]);
+                sweep_decide<MODEL, D>(d, c, y, pc, ctr, ys, ylev, L, it.dt, it.counter, false, err, it.st, it.remesh);
+            }
+        }
+        __syncthreads();
+    }
+}
+
+template <int MODEL, int D>
+struct OpSweepRest {
+    __device__ __forceinline__ void operator()(const Item &it, const MConst &c, int li) const {
+        sweep_body<MODEL, D>(it.d, c, it.rest, it.st->n_rest2, li, it.L, it.dt, it.counter, 0, it.st, it.remesh);
+    }
+};
 
 template <int MODEL, int D>
 struct OpUpdate {
