@@ -24,10 +24,32 @@ sys.path.insert(0, ROOT)
 
 B_ALG = 8 * (2 * 5 + 2)  # bytes per leaf-cell update: state in + out, cell time in + out (euler<3>: NE = 5)
 HBM_PEAK_GBS = 8000.0
-# HBM bytes per leaf-cell update of all substep kernels on this workload, from
-# two rocprofv3 counter runs (FETCH_SIZE x 2.00, WRITE_SIZE x 1.00, calibrated
-# on a 1 GiB device copy): profiles/leaf_update_euler3d_256_hbm_bytes.txt
-MEASURED_BYTES_PER_UPDATE = 1029.0
+FP64_PEAK_TFLOPS = 78.6
+# 1024 SIMDs x 2.4 GHz, one wave64 VALU instruction per 4 cycles, 64 lanes
+VALU_PEAK_TLANE = 1024 * 2.4e9 / 4 * 64 / 1e12
+
+
+def profile_figures():
+    """Per-update figures of the box workload from the committed rocprofv3
+    summaries: measured HBM bytes (FETCH_SIZE / WRITE_SIZE, calibrated on a
+    1 GiB device copy), FP64 FLOPs and VALU instructions per leaf-cell update.
+    None where a summary is missing."""
+    out = {"bytes": None, "flops": None, "valu": None}
+    prof = os.path.join(ROOT, "profiles")
+    try:
+        with open(os.path.join(prof, "leaf_update_euler3d_256_hbm_bytes.txt")) as f:
+            for line in f:
+                if line.startswith("Measured HBM bytes per leaf-cell update:"):
+                    out["bytes"] = float(line.split(":")[1].split()[0])
+        with open(os.path.join(prof, "leaf_update_euler3d_256_valu.txt")) as f:
+            for line in f:
+                if "FP64 FLOPs per leaf-cell update" in line:
+                    out["flops"] = float(line.rsplit(":", 1)[1])
+                if "VALU instructions per leaf-cell update" in line:
+                    out["valu"] = float(line.rsplit(":", 1)[1])
+    except OSError:
+        pass
+    return out
 
 
 def cpu_baseline(case, tree, seconds_target=20.0):
@@ -526,6 +548,9 @@ def main():
     alg_bytes_per_launch = B_ALG * updates / max(launches, 1)
     avg_ms = kern_ms / max(launches, 1)
     achieved = alg_bytes_per_launch / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
+    # leaf-cell updates per second of substep kernel time (the roofs' rate)
+    upd_rate = updates / (kern_ms * 1e-3) if kern_ms > 0 else 0.0
+    pf = profile_figures()
     g.close()
     base = None
     if not args.no_cpu_baseline:
@@ -551,10 +576,19 @@ def main():
                    "cell_updates_timed": updates, "substeps_timed": launches},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": MEASURED_BYTES_PER_UPDATE * updates / max(launches, 1),
+                     "traffic": pf["bytes"] * updates / max(launches, 1) if pf["bytes"] else None,
                      "kernel": "one level substep (list selection, flux, update, sweep, LTS apply, histogram)",
-                     "bytes_per_update": B_ALG, "measured_bytes_per_update": MEASURED_BYTES_PER_UPDATE,
-                     "avg_launch_ms": avg_ms},
+                     "bytes_per_update": B_ALG, "measured_bytes_per_update": pf["bytes"],
+                     "avg_launch_ms": avg_ms,
+                     # secondary roofs of the same substep kernels: FP64 arithmetic and VALU issue
+                     "fp64": ({"achieved_tflops": pf["flops"] * upd_rate / 1e12, "peak_tflops": FP64_PEAK_TFLOPS,
+                               "frac": pf["flops"] * upd_rate / 1e12 / FP64_PEAK_TFLOPS,
+                               "flops_per_update": pf["flops"]} if pf["flops"] else None),
+                     "valu_issue": ({"achieved_tlane_instr": pf["valu"] * upd_rate / 1e12,
+                                     "peak_tlane_instr": VALU_PEAK_TLANE,
+                                     "frac": pf["valu"] * upd_rate / 1e12 / VALU_PEAK_TLANE,
+                                     "instr_per_update": pf["valu"]} if pf["valu"] else None),
+                     "figures_from": "profiles/leaf_update_euler3d_256_{hbm_bytes,valu}.txt"},
         "cpu_baseline": base,
     }
     print(json.dumps(out), flush=True)

@@ -13,6 +13,7 @@
 
 #include "model.hpp"
 
+
 namespace dacti {
 
 enum : uint8_t {
@@ -1833,27 +1834,59 @@ struct OpDepositCoarse {
 // consecutive node ids: a 4x4x4 (8x8) Morton block of a breadth-first
 // level.  A face between two same-level leaves of the window that both
 // update in this substep is evaluated once, by its left cell, and the
-// right cell takes the flux from LDS; every other face a cell needs is
-// evaluated in one compacted pass over the window.  Which side evaluates a
-// face, the fluxes received from finer neighbours, the deposits for
-// neighbours on other LTS levels and the per-cell face order of the sum
-// are flux_simple_body's (impl/flux.hpp:32-91), so the results equal that
-// op's bit for bit.  Cells with a face to another tree level go to the
-// rest list, as there.  One instance (items[0]); the windows cover the ids
-// from the cluster list's first to its last entry, aligned to it.walign.
+// right cell takes the flux from the left cell's lane; every other face a
+// cell needs is evaluated in one compacted pass over the window.  Which
+// side evaluates a face, the fluxes received from finer neighbours, the
+// deposits for neighbours on other LTS levels and the per-cell face order
+// of the sum are flux_simple_body's (impl/flux.hpp:32-91), so the results
+// equal that op's bit for bit.  Cells with a face to another tree level go
+// to the rest list, as there.  One instance (items[0]); the windows cover
+// the ids from the cluster list's first to its last entry, aligned to
+// it.walign.
 enum : int { WK_NONE = 0, WK_BOUNDARY = 1, WK_OWN = 2, WK_NEIGHBOUR = 3, WK_SHARED = 4 };
 
+// The - face flux of cell z along dim in the compacted pass: a boundary
+// face, a face z owns toward a slower neighbour (with the deposit for a
+// neighbour outside the cluster), or a face z's - neighbour owns.
 template <int MODEL, int D>
-// three waves per SIMD (12.8 KB of LDS per wave allows no more): a few
-// spilled registers cost less than the latency hiding of the third wave
+__device__ __forceinline__ void window_minus_flux(const Dev &d, const MConst &c, int32_t z, int dim, int kind,
+                                                  int dep, double dt, double *F) {
+    using S = Sz<MODEL, D>;
+    constexpr int NF = S::NF;
+    const int64_t C = d.C;
+    if (kind == WK_BOUNDARY) {
+        boundary_flux<MODEL, D>(d, c, z, 2 * dim, dim, 0, dt, F);
+        return;
+    }
+    const int32_t nb = d.nbr[(2 * dim) * C + z];
+    Side<MODEL, D> sz, sn;
+    load_side<MODEL, D>(d, z, dim, sz);
+    load_side<MODEL, D>(d, nb, dim, sn);
+    const double zc = d.center[dim * C + z], nc = d.center[dim * C + nb], zs = d.size[z];
+    const int zlev = d.level[z];
+    if (kind == WK_OWN) {
+        face_flux_sides_body<MODEL, D>(c, sz, zc, zs, zlev, sn, nc, dim, 0, dt, F);
+        if (dep) {
+#pragma unroll
+            for (int e = 0; e < NF; ++e) {
+                GMEM double &a = d.acc[((2 * dim + 1) * NF + e) * C + nb];
+                a = a - F[e];
+            }
+        }
+    } else {
+        face_flux_sides_body<MODEL, D>(c, sn, nc, zs, zlev, sz, zc, dim, 1, dt, F);
+    }
+}
+
+// Neighbour ids and the + face fluxes move between the window's lanes by
+// lane permutes; LDS holds only the compacted pass (queue and results).
+// Three waves per SIMD: the register budget of a fourth (128 VGPRs) spills
+// and ran 10% slower, as did four waves of the sweep.
+template <int MODEL, int D>
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) k_flux_window(const Item *__restrict__ items, const MConst *__restrict__ mcs) {
     using S = Sz<MODEL, D>;
     constexpr int NF = S::NF, NE = S::NE;
-    __shared__ double X[NF][64];     // + face flux of the current axis, evaluated by the lane's cell
     __shared__ double M[D][NF][64];  // - face flux evaluated in the compacted pass
-    __shared__ int32_t PN[D][64];    // + neighbour of the lane's cell
-    __shared__ int32_t NM[D][64];    // - neighbour of the lane's cell
-    __shared__ int32_t CP[64];       // bit dim: the lane's cell evaluates its + face
     __shared__ int32_t Q[D * 64];    // compacted pass: lane | dim << 6 | kind << 8 | deposit << 11
     const Item &it = items[0];
     const MConst &c = mcs[0];
@@ -1874,11 +1907,13 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) k_
         const bool member = yid >= 1 && yid < it.n && d.cluster[yid] == L;
         const int32_t y = (int32_t) yid;
         int32_t nbs[2 * D];
-        int nac[2 * D], ncls[2 * D];
+        // face f's neighbour: ACTI level in bits 4f..4f+3, cluster + 1 in
+        // bits 5f..5f+4 (cluster is -1..15)
+        uint32_t nacp = 0, nclp = 0;
         bool part = false;
         int ylev = 0;
 #pragma unroll
-        for (int f = 0; f < 2 * D; ++f) nbs[f] = 0, nac[f] = 0, ncls[f] = -1;
+        for (int f = 0; f < 2 * D; ++f) nbs[f] = 0;
         if (member) {
             ylev = d.level[y];
             bool simple = true;
@@ -1886,9 +1921,11 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) k_
             for (int f = 0; f < 2 * D; ++f) {
                 const int32_t nb = d.nbr[f * C + y];
                 nbs[f] = nb;
-                nac[f] = nb ? d.acti[nb] : 0;
-                ncls[f] = nb ? d.cluster[nb] : -1;
-                if (nb != 0 && (!(d.flags[nb] & F_LEAF) || d.level[nb] != ylev)) simple = false;
+                if (nb) {
+                    nacp |= (uint32_t) d.acti[nb] << (4 * f);
+                    nclp |= (uint32_t) (d.cluster[nb] + 1) << (5 * f);
+                    if (!(d.flags[nb] & F_LEAF) || d.level[nb] != ylev) simple = false;
+                }
             }
             if (!simple) it.rest[atomicAdd(&it.st->n_rest, 1)] = y;
             part = simple;
@@ -1898,33 +1935,34 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) k_
         // a finer LTS level (whose deposit is received in the sum)
         int cp = 0;
 #pragma unroll
-        for (int dim = 0; dim < D; ++dim) {
-            PN[dim][lane] = nbs[2 * dim + 1];
-            NM[dim][lane] = nbs[2 * dim];
-            if (part && (nbs[2 * dim + 1] == 0 || nac[2 * dim + 1] <= L)) cp |= 1 << dim;
-        }
-        CP[lane] = cp;
-        __syncthreads();
+        for (int dim = 0; dim < D; ++dim)
+            if (part && (nbs[2 * dim + 1] == 0 || (int) ((nacp >> (4 * (2 * dim + 1))) & 15) <= L)) cp |= 1 << dim;
         // - faces: the left cell's evaluation when it is a window lane that
         // evaluates the face, else queued for the compacted pass
         int kinds = 0;  // 3 bits per axis
         int qn = 0;
 #pragma unroll
         for (int dim = 0; dim < D; ++dim) {
+            const int f = 2 * dim;
+            const int32_t nbm = nbs[f];
+            const int64_t o64 = (int64_t) nbm - base;
+            const bool inr = nbm != 0 && o64 >= 0 && o64 < 64;
+            const int o = inr ? (int) o64 : lane;
+            // the window neighbour's + neighbour and + face decisions (every lane takes part)
+            const int32_t nxt_w = __shfl(nbs[f + 1], o);
+            const int cp_w = __shfl(cp, o);
             int kind = WK_NONE, dep = 0;
             if (part) {
-                const int f = 2 * dim;
-                const int32_t nb = nbs[f];
-                if (nb == 0) {
+                const int nac = (int) ((nacp >> (4 * f)) & 15), ncl = (int) ((nclp >> (5 * f)) & 31) - 1;
+                if (nbm == 0) {
                     kind = WK_BOUNDARY;
-                } else if (nac[f] < L) {
+                } else if (nac < L) {
                     kind = WK_OWN;
-                    dep = ncls[f] != L;
-                } else if (ncls[f] == L) {
-                    const int64_t o = (int64_t) nb - base;
-                    const bool inwin = o >= 0 && o < 64 && ((pm >> o) & 1);
-                    const int32_t nxt = inwin ? PN[dim][o] : d.nbr[(2 * dim + 1) * C + nb];
-                    if (nxt == y) kind = inwin && ((CP[o] >> dim) & 1) ? WK_SHARED : WK_NEIGHBOUR;
+                    dep = ncl != L;
+                } else if (ncl == L) {
+                    const bool inwin = inr && ((pm >> o) & 1);
+                    const int32_t nxt = inwin ? nxt_w : d.nbr[(f + 1) * C + nbm];
+                    if (nxt == y) kind = inwin && ((cp_w >> dim) & 1) ? WK_SHARED : WK_NEIGHBOUR;
                 }
             }
             kinds |= kind << (3 * dim);
@@ -1936,34 +1974,13 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) k_
         __syncthreads();
         for (int j = lane; j < qn; j += 64) {
             const int q = Q[j];
-            const int zl = q & 63, dim = (q >> 6) & 3, kind = (q >> 8) & 7;
-            const int32_t z = (int32_t) (base + zl);
+            const int zl = q & 63, dim = (q >> 6) & 3;
             double F[NF];
-            if (kind == WK_BOUNDARY) {
-                boundary_flux<MODEL, D>(d, c, z, 2 * dim, dim, 0, dt, F);
-            } else {
-                const int32_t nb = NM[dim][zl];
-                Side<MODEL, D> sz, sn;
-                load_side<MODEL, D>(d, z, dim, sz);
-                load_side<MODEL, D>(d, nb, dim, sn);
-                const double zc = d.center[dim * C + z], nc = d.center[dim * C + nb], zs = d.size[z];
-                const int zlev = d.level[z];
-                if (kind == WK_OWN) {
-                    face_flux_sides_body<MODEL, D>(c, sz, zc, zs, zlev, sn, nc, dim, 0, dt, F);
-                    if ((q >> 11) & 1) {
-#pragma unroll
-                        for (int e = 0; e < NF; ++e) {
-                            GMEM double &a = d.acc[((2 * dim + 1) * NF + e) * C + nb];
-                            a = a - F[e];
-                        }
-                    }
-                } else {
-                    face_flux_sides_body<MODEL, D>(c, sn, nc, zs, zlev, sz, zc, dim, 1, dt, F);
-                }
-            }
+            window_minus_flux<MODEL, D>(d, c, (int32_t) (base + zl), dim, (q >> 8) & 7, (q >> 11) & 1, dt, F);
 #pragma unroll
             for (int e = 0; e < NF; ++e) M[dim][e][zl] = F[e];
         }
+        __syncthreads();
         // + faces axis by axis; after each axis the cell adds its - and +
         // face in its face order
         double ys = 0.0, yc = 0.0;
@@ -1991,19 +2008,21 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) k_
         for (int dim = 0; dim < D; ++dim) {
             const int fm = 2 * dim, fp = 2 * dim + 1;
             const int32_t nbp = pick_i<2 * D>(nbs, fp);
+            double F[NF];
+#pragma unroll
+            for (int e = 0; e < NF; ++e) F[e] = 0.0;
             if ((cp >> dim) & 1) {
 #pragma unroll
                 for (int e = 0; e < NE; ++e) sy.s[e] = d.slope[(dim * NE + e) * C + y];
                 if constexpr (MODEL == MULTI) sy.kn = d.k[(S::IKN + dim) * C + y];
                 yc = d.center[dim * C + y];
-                double F[NF];
                 if (nbp == 0) {
                     boundary_flux<MODEL, D>(d, c, y, fp, dim, 1, dt, F);
                 } else {
                     Side<MODEL, D> sn;
                     load_side<MODEL, D>(d, nbp, dim, sn);
                     face_flux_sides_body<MODEL, D>(c, sy, yc, ys, ylev, sn, d.center[dim * C + nbp], dim, 1, dt, F);
-                    if (pick_i<2 * D>(ncls, fp) != L) {
+                    if ((int) ((nclp >> (5 * fp)) & 31) - 1 != L) {
 #pragma unroll
                         for (int e = 0; e < NF; ++e) {
                             GMEM double &a = d.acc[(fm * NF + e) * C + nbp];
@@ -2011,15 +2030,16 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) k_
                         }
                     }
                 }
-#pragma unroll
-                for (int e = 0; e < NF; ++e) X[e][lane] = F[e];
             }
-            __syncthreads();
+            // the - neighbour's + face flux, from its lane (every lane takes part)
+            const int32_t nbm = pick_i<2 * D>(nbs, fm);
+            const int kind = (kinds >> (3 * dim)) & 7;
+            const int om = kind == WK_SHARED ? (int) ((int64_t) nbm - base) : lane;
+            double X[NF];
+#pragma unroll
+            for (int e = 0; e < NF; ++e) X[e] = __shfl(F[e], om);
             if (part) {
-                const int32_t nbm = pick_i<2 * D>(nbs, fm);
-                const int kind = (kinds >> (3 * dim)) & 7;
-                const bool takem = nbm != 0 && pick_i<2 * D>(nac, fm) > L;
-                const int o = (int) ((int64_t) nbm - base);
+                const bool takem = nbm != 0 && (int) ((nacp >> (4 * fm)) & 15) > L;
                 double cc[NF];
 #pragma unroll
                 for (int e = 0; e < NF; ++e) {
@@ -2031,7 +2051,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) k_
                 }
                 if (kind == WK_SHARED) {
 #pragma unroll
-                    for (int e = 0; e < NF; ++e) cc[e] = cc[e] + X[e][o];
+                    for (int e = 0; e < NF; ++e) cc[e] = cc[e] + X[e];
                 } else if (kind == WK_BOUNDARY) {
 #pragma unroll
                     for (int e = 0; e < NF; ++e) cc[e] = cc[e] - M[dim][e][lane];
@@ -2041,7 +2061,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) k_
                 }
 #pragma unroll
                 for (int e = 0; e < NF; ++e) tot[e] = tot[e] + cc[e];
-                const bool takep = nbp != 0 && pick_i<2 * D>(nac, fp) > L;
+                const bool takep = nbp != 0 && (int) ((nacp >> (4 * fp)) & 15) > L;
 #pragma unroll
                 for (int e = 0; e < NF; ++e) {
                     cc[e] = 0.0;
@@ -2052,17 +2072,18 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) k_
                 }
                 if ((cp >> dim) & 1) {
 #pragma unroll
-                    for (int e = 0; e < NF; ++e) cc[e] = cc[e] - X[e][lane];
+                    for (int e = 0; e < NF; ++e) cc[e] = cc[e] - F[e];
                 }
 #pragma unroll
                 for (int e = 0; e < NF; ++e) tot[e] = tot[e] + cc[e];
             }
-            __syncthreads();
         }
         if (part) {
 #pragma unroll
             for (int e = 0; e < NF; ++e) d.tot[e * C + y] = tot[e];
         }
+        // the next window's queue and results reuse the LDS
+        __syncthreads();
     }
 }
 
