@@ -52,6 +52,43 @@ def profile_figures():
     return out
 
 
+B_ALG_MULTI2 = 8 * (2 * 6 + 2) + 16  # multi<2>: state in + out, time in + out, key/level/flags
+B_ALG_MULTI3 = 8 * (2 * 7 + 2) + 16  # multi<3>
+
+
+def substep_roofline(b_alg, updates, launches, kern_ms, kernel):
+    """HBM roofline of one level substep (all its kernels) from the HIP-event
+    kernel time: algorithmic bytes per launch / average launch duration."""
+    avg_ms = kern_ms / max(launches, 1)
+    achieved = b_alg * updates / max(launches, 1) / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
+    return {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "traffic": None, "bytes_per_update": b_alg, "kernel": kernel,
+            "avg_launch_ms": avg_ms, "launches": launches}
+
+
+def oracle_instances_baseline(fam, trees, seconds=20.0):
+    """The oracle (OpenMP) stepping the batch's instances one after another
+    from the same initial trees, one global step each, until about `seconds`
+    of CPU work: a bounded sample of the batched workload."""
+    from oracle.binding import Oracle
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    n = k = 0
+    el = 0.0
+    for (case, _), tree in zip(fam, trees):
+        o = Oracle(case, tree)
+        t0 = time.perf_counter()
+        o.integrate()
+        el += time.perf_counter() - t0
+        n += o.cell_updates
+        k += 1
+        o.close()
+        if el >= seconds:
+            break
+    return {"value": n / el, "unit": "cell-updates/s", "cores": threads, "kind": "port",
+            "sample": f"the first global step of the first {k} instances from the same initial states "
+                      f"({n} leaf-cell updates in {el:.1f} s, OpenMP {threads} threads)"}
+
+
 def cpu_baseline(case, tree, seconds_target=20.0):
     """The oracle (C restatement, OpenMP) on the same initial tree: one global step."""
     from oracle.binding import Oracle
@@ -129,6 +166,7 @@ def batched(args):
         b.integrate()
         print(f"[bench] batched warmup/tuning step {w} done", file=sys.stderr, flush=True)
     tuned = b.tune_results()
+    b.set_timing(True)
     s0 = b.stats()
     t0 = time.perf_counter()
     for k in range(args.steps):
@@ -141,6 +179,12 @@ def batched(args):
     same = all(np.array_equal(a.get_nodes()["u"], m.get_nodes()["u"], equal_nan=True)
                for a, m in zip(plain, members))
     blk, grd, gr = b.launch()
+    roof = substep_roofline(B_ALG_MULTI2, up, s1.flux_launches - s0.flux_launches,
+                            s1.substep_kernel_ms - s0.substep_kernel_ms,
+                            "one level substep of all instances (head, remesh and tail sequences)")
+    base = None
+    if not args.no_cpu_baseline:
+        base = oracle_instances_baseline(fam, trees)
     out = {
         "metric": "cell-updates/s (HLLC+MUSCL FP64) & %HBM roofline; learned/batched vs plain",
         "value": up / el,
@@ -160,8 +204,8 @@ def batched(args):
         "batched_over_plain": (up / el) / (up_plain / el_plain),
         "same_state_as_plain": bool(same),
         "instances_stopped_nan": int(len(plain) - sum(alive)),
-        "roofline": None,
-        "cpu_baseline": None,
+        "roofline": roof,
+        "cpu_baseline": base,
     }
     b.close()
     for g in plain + members:
@@ -356,6 +400,7 @@ def bubble(args):
         g.set_launch(block=128, grid=2048, graphs=True)
     for w in range(args.warmup):
         g.integrate()
+    g.set_timing(True)
     s0 = g.stats()
     t0 = time.perf_counter()
     for k in range(args.steps):
@@ -390,7 +435,9 @@ def bubble(args):
                                "remesh every substep, LTS on",
                    "nodes": s1.n_nodes, "active_leaves": s1.n_active, "max_acti_level": s1.max_acti_level,
                    "cell_updates_timed": updates, "graphs": bool(args.graphs)},
-        "roofline": None,
+        "roofline": substep_roofline(B_ALG_MULTI2, updates, s1.flux_launches - s0.flux_launches,
+                                     s1.substep_kernel_ms - s0.substep_kernel_ms,
+                                     "one level substep (head, remesh and tail sequences)"),
         "cpu_baseline": base,
     }
     g.close()
@@ -420,11 +467,9 @@ def drop(args):
     el = time.perf_counter() - t0
     s1 = g.stats()
     updates = s1.cell_updates - s0.cell_updates
-    b_alg = 8 * (2 * 7 + 2) + 16  # multi<3>: state in + out, time in + out, key/level/flags
     launches = s1.flux_launches - s0.flux_launches
-    kern_ms = s1.substep_kernel_ms - s0.substep_kernel_ms
-    avg_ms = kern_ms / max(launches, 1)
-    achieved = b_alg * updates / max(launches, 1) / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
+    roof = substep_roofline(B_ALG_MULTI3, updates, launches, s1.substep_kernel_ms - s0.substep_kernel_ms,
+                            "one level substep (head, remesh and tail sequences)")
     base = None
     if not args.no_cpu_baseline:
         from oracle.binding import Oracle
@@ -446,9 +491,7 @@ def drop(args):
                                f"{args.base + args.extra}, remesh every substep, LTS on",
                    "nodes": s1.n_nodes, "active_leaves": s1.n_active, "max_acti_level": s1.max_acti_level,
                    "cell_updates_timed": updates, "substeps_timed": launches},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "bytes_per_update": b_alg,
-                     "kernel": "one level substep", "avg_launch_ms": avg_ms},
+        "roofline": roof,
         "cpu_baseline": base,
     }
     g.close()

@@ -251,6 +251,7 @@ struct BOps {
         rc = upload_tables(b, changed);  // new pool sizes and pool tiles
         if (rc) return rc;
         // new-block offsets, splits, unsplits
+        if (b->timing && hipEventRecord(b->ev[4], st) != hipSuccess) return DACTI_E_HIP;
         const unsigned gr = grid_for(b, tr);
         k_rn_count<<<gr, 256, 0, st>>>(b->d_items, b->d_toffr, n, b->d_tilecnt);
         k_sel_scan<2><<<1, 1024, 0, st>>>(b->d_items, b->d_toffr, n, b->d_tilecnt);
@@ -266,6 +267,7 @@ struct BOps {
         k_sel_scatter<1><<<gp, 256, 0, st>>>(b->d_items, b->d_toffp, n, b->d_tilecnt);
         k_batch<<<grid_for(b, t2), B, 0, st>>>(b->d_items, b->d_mcs, b->d_toff2, n, OpReassign<MODEL, D>{});
         HIPCHK(b, hipGetLastError());
+        if (b->timing && hipEventRecord(b->ev[5], st) != hipSuccess) return DACTI_E_HIP;
         return DACTI_OK;
     }
 
@@ -301,6 +303,7 @@ struct BOps {
         }
         b->h_toff[n] = (int) tl;
         int rc = 0;
+        bool remeshed = false;
         if (any_changed) {
             rc = upload_tables(b, changed);
             if (rc) return rc;
@@ -371,6 +374,7 @@ struct BOps {
                 // uploads the grown pools' items and pool tiles itself
                 rc = remesh_all(b, L, marked, changed);
                 if (rc) return rc;
+                remeshed = true;
             }
             any_list = false;
             for (int i = 0; i < n; ++i) any_list |= b->h_items[i].nl > 0;
@@ -384,12 +388,13 @@ struct BOps {
             if (rc) return rc;
         }
         if (b->timing && any_list && any_changed) {
-            float a = 0, c = 0;
+            float a = 0, c = 0, m = 0;
             if (hipEventElapsedTime(&a, b->ev[0], b->ev[1]) != hipSuccess ||
-                hipEventElapsedTime(&c, b->ev[2], b->ev[3]) != hipSuccess)
+                hipEventElapsedTime(&c, b->ev[2], b->ev[3]) != hipSuccess ||
+                (remeshed && hipEventElapsedTime(&m, b->ev[4], b->ev[5]) != hipSuccess))
                 return DACTI_E_HIP;
             b->flux_ms += a;
-            b->substep_ms += a + c;
+            b->substep_ms += a + c + m;
         }
         for (int i = 0; i < n; ++i) {
             dacti_handle *h = b->hs[(size_t) i];

@@ -126,7 +126,8 @@ struct dacti_batch {
     hipGraphExec_t g_head_dense = nullptr;  // head with the window flux op (a dense cluster of a batch of one)
     bool dense = false;                     // this substep takes the window flux op
     bool timing = false;
-    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    // head start/end, tail start/end, remesh start/end
+    hipEvent_t ev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
     double flux_ms = 0, substep_ms = 0;
     int64_t cell_updates = 0, n_substeps = 0, n_global_steps = 0, launches = 0;
     int top = 15;  // no cluster above this level in any instance
