@@ -40,7 +40,8 @@ struct BOps {
         const unsigned gl = list_grid(b), gp = pool_grid(b);
         const int B = b->block, n = b->n;
         Item *it = b->d_items;
-        k_sel_count<0><<<gp, 256, 0, st>>>(it, b->d_toffp, n, b->d_tilecnt);
+        if (b->dense) k_sel_count<0, true><<<gp, 256, 0, st>>>(it, b->d_toffp, n, b->d_tilecnt);
+        else k_sel_count<0><<<gp, 256, 0, st>>>(it, b->d_toffp, n, b->d_tilecnt);
         k_sel_scan<0><<<1, 1024, 0, st>>>(it, b->d_toffp, n, b->d_tilecnt);
         k_sel_scatter<0><<<gp, 256, 0, st>>>(it, b->d_toffp, n, b->d_tilecnt);
         if (b->dense) k_flux_window<MODEL, D><<<window_grid(b), 64, 0, st>>>(it, b->d_mcs);

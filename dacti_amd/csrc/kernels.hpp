@@ -39,6 +39,8 @@ struct Dev {
     GMEM double *scratch;                     // C doubles (smoothing / reductions)
     GMEM int32_t *err;                        // device error word (subtree deeper than supported)
     GMEM int32_t *brec;                       // object boundary record of a node's BC_OBJECT faces
+    GMEM uint32_t *meta;                      // flags | level << 8 | acti << 16 | cluster << 24, written by the
+                                              // list selection of a dense substep for the window flux op
     const GMEM BndRec *brecs;                 // the records (fixed at construction)
 };
 
@@ -1845,43 +1847,35 @@ struct OpDepositCoarse {
 // it.walign.
 enum : int { WK_NONE = 0, WK_BOUNDARY = 1, WK_OWN = 2, WK_NEIGHBOUR = 3, WK_SHARED = 4 };
 
-// The - face flux of cell z along dim in the compacted pass: a boundary
-// face, a face z owns toward a slower neighbour (with the deposit for a
-// neighbour outside the cluster), or a face z's - neighbour owns.
+// a window lane's cell side from lane o (every lane takes part)
 template <int MODEL, int D>
-__device__ __forceinline__ void window_minus_flux(const Dev &d, const MConst &c, int32_t z, int dim, int kind,
-                                                  int dep, double dt, double *F) {
+__device__ __forceinline__ void shfl_side(const Side<MODEL, D> &a, int o, Side<MODEL, D> &b) {
     using S = Sz<MODEL, D>;
-    constexpr int NF = S::NF;
-    const int64_t C = d.C;
-    if (kind == WK_BOUNDARY) {
-        boundary_flux<MODEL, D>(d, c, z, 2 * dim, dim, 0, dt, F);
-        return;
-    }
-    const int32_t nb = d.nbr[(2 * dim) * C + z];
-    Side<MODEL, D> sz, sn;
-    load_side<MODEL, D>(d, z, dim, sz);
-    load_side<MODEL, D>(d, nb, dim, sn);
-    const double zc = d.center[dim * C + z], nc = d.center[dim * C + nb], zs = d.size[z];
-    const int zlev = d.level[z];
-    if (kind == WK_OWN) {
-        face_flux_sides_body<MODEL, D>(c, sz, zc, zs, zlev, sn, nc, dim, 0, dt, F);
-        if (dep) {
 #pragma unroll
-            for (int e = 0; e < NF; ++e) {
-                GMEM double &a = d.acc[((2 * dim + 1) * NF + e) * C + nb];
-                a = a - F[e];
-            }
-        }
-    } else {
-        face_flux_sides_body<MODEL, D>(c, sn, nc, zs, zlev, sz, zc, dim, 1, dt, F);
+    for (int e = 0; e < S::NE; ++e) {
+        b.p[e] = __shfl(a.p[e], o);
+        b.s[e] = __shfl(a.s[e], o);
+        b.r[e] = __shfl(a.r[e], o);
+    }
+    b.t = __shfl(a.t, o);
+    if constexpr (MODEL == MULTI) {
+        b.kap = __shfl(a.kap, o);
+        b.kn = __shfl(a.kn, o);
+    }
+    if constexpr (S::NG > 0) {
+#pragma unroll
+        for (int x = 0; x < D; ++x)
+#pragma unroll
+            for (int e = 0; e < S::NG; ++e) b.g[x][e] = __shfl(a.g[x][e], o);
     }
 }
 
-// Neighbour ids and the + face fluxes move between the window's lanes by
-// lane permutes; LDS holds only the compacted pass (queue and results).
-// Three waves per SIMD: the register budget of a fourth (128 VGPRs) spills
-// and ran 10% slower, as did four waves of the sweep.
+// Every lane holds its own node's state, time and (per axis) slope, so the
+// side of a neighbour inside the window comes from that neighbour's lane
+// (lane permutes), and so do its metadata word and the + face fluxes; only
+// neighbours outside the window are read from memory.  LDS holds only the
+// compacted pass (queue and results).  Three waves per SIMD: the register
+// budget of a fourth (128 VGPRs) spills and ran 10% slower.
 template <int MODEL, int D>
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) k_flux_window(const Item *__restrict__ items, const MConst *__restrict__ mcs) {
     using S = Sz<MODEL, D>;
@@ -1904,32 +1898,53 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) k_
     for (int k = (int) blockIdx.x; k < nw; k += (int) gridDim.x) {
         const int64_t base = al + (w0 + xcd_block(k, nw)) * 64;
         const int64_t yid = base + lane;
-        const bool member = yid >= 1 && yid < it.n && d.cluster[yid] == L;
+        const bool valid = yid >= 1 && yid < it.n;
         const int32_t y = (int32_t) yid;
-        int32_t nbs[2 * D];
-        // face f's neighbour: ACTI level in bits 4f..4f+3, cluster + 1 in
-        // bits 5f..5f+4 (cluster is -1..15)
-        uint32_t nacp = 0, nclp = 0;
-        bool part = false;
-        int ylev = 0;
+        // the lane's own node: metadata, state, time
+        const uint32_t ymeta = valid ? d.meta[y] : 0u;
+        const bool member = valid && ((ymeta >> 24) & 0xff) == (uint32_t) L;
+        const int ylev = (int) ((ymeta >> 8) & 0xff);
+        Side<MODEL, D> sy;
+        double ys = 0.0;
+        if (valid) {
+            ys = d.size[y];
 #pragma unroll
-        for (int f = 0; f < 2 * D; ++f) nbs[f] = 0;
-        if (member) {
-            ylev = d.level[y];
-            bool simple = true;
-#pragma unroll
-            for (int f = 0; f < 2 * D; ++f) {
-                const int32_t nb = d.nbr[f * C + y];
-                nbs[f] = nb;
-                if (nb) {
-                    nacp |= (uint32_t) d.acti[nb] << (4 * f);
-                    nclp |= (uint32_t) (d.cluster[nb] + 1) << (5 * f);
-                    if (!(d.flags[nb] & F_LEAF) || d.level[nb] != ylev) simple = false;
-                }
+            for (int e = 0; e < NE; ++e) {
+                sy.p[e] = d.p[e * C + y];
+                sy.r[e] = d.rhs[e * C + y];
             }
-            if (!simple) it.rest[atomicAdd(&it.st->n_rest, 1)] = y;
-            part = simple;
+            sy.t = d.t[y];
+            if constexpr (MODEL == MULTI) sy.kap = d.k[S::IKAPPA * C + y];
+            if constexpr (S::NG > 0) {
+#pragma unroll
+                for (int a = 0; a < D; ++a)
+#pragma unroll
+                    for (int e = 0; e < S::NG; ++e) sy.g[a][e] = d.grad[(a * S::NGX + e) * C + y];
+            }
         }
+        int32_t nbs[2 * D];
+#pragma unroll
+        for (int f = 0; f < 2 * D; ++f) nbs[f] = member ? d.nbr[f * C + y] : 0;
+        // face f's neighbour: ACTI level in bits 4f..4f+3, cluster + 1 in
+        // bits 5f..5f+4 (cluster is -1..15); its metadata from its lane when
+        // it lies in the window
+        uint32_t nacp = 0, nclp = 0;
+        bool simple = true;
+#pragma unroll
+        for (int f = 0; f < 2 * D; ++f) {
+            const int32_t nb = nbs[f];
+            const int64_t o64 = (int64_t) nb - base;
+            const bool inr = nb != 0 && o64 >= 0 && o64 < 64;
+            uint32_t mw = __shfl(ymeta, inr ? (int) o64 : lane);
+            if (nb != 0 && !inr) mw = d.meta[nb];
+            if (nb) {
+                nacp |= ((mw >> 16) & 15u) << (4 * f);
+                nclp |= ((((mw >> 24) & 0xff) + 1) & 31u) << (5 * f);
+                if (!(mw & F_LEAF) || (int) ((mw >> 8) & 0xff) != ylev) simple = false;
+            }
+        }
+        if (member && !simple) it.rest[atomicAdd(&it.st->n_rest, 1)] = y;
+        const bool part = member && simple;
         const uint64_t pm = __ballot(part);
         // a + face is evaluated by the lane's cell unless the neighbour is on
         // a finer LTS level (whose deposit is received in the sum)
@@ -1972,56 +1987,96 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) k_
             qn += __popcll(qm);
         }
         __syncthreads();
-        for (int j = lane; j < qn; j += 64) {
-            const int q = Q[j];
+        for (int j0 = 0; j0 < qn; j0 += 64) {
+            const int j = j0 + lane;
+            const bool act = j < qn;
+            const int q = act ? Q[j] : lane;
             const int zl = q & 63, dim = (q >> 6) & 3;
-            double F[NF];
-            window_minus_flux<MODEL, D>(d, c, (int32_t) (base + zl), dim, (q >> 8) & 7, (q >> 11) & 1, dt, F);
+            // the queued cell's state and time from its lane (every lane takes part)
+            Side<MODEL, D> sz;
 #pragma unroll
-            for (int e = 0; e < NF; ++e) M[dim][e][zl] = F[e];
+            for (int e = 0; e < NE; ++e) {
+                sz.p[e] = __shfl(sy.p[e], zl);
+                sz.r[e] = __shfl(sy.r[e], zl);
+            }
+            sz.t = __shfl(sy.t, zl);
+            if (act) {
+                const int kind = (q >> 8) & 7;
+                const int32_t z = (int32_t) (base + zl);
+                double F[NF];
+                if (kind == WK_BOUNDARY) {
+                    boundary_flux<MODEL, D>(d, c, z, 2 * dim, dim, 0, dt, F);
+                } else {
+                    const int32_t nb = d.nbr[(2 * dim) * C + z];
+#pragma unroll
+                    for (int e = 0; e < NE; ++e) sz.s[e] = d.slope[(dim * NE + e) * C + z];
+                    if constexpr (MODEL == MULTI) {
+                        sz.kap = d.k[S::IKAPPA * C + z];
+                        sz.kn = d.k[(S::IKN + dim) * C + z];
+                    }
+                    if constexpr (S::NG > 0) {
+#pragma unroll
+                        for (int a = 0; a < D; ++a)
+#pragma unroll
+                            for (int e = 0; e < S::NG; ++e) sz.g[a][e] = d.grad[(a * S::NGX + e) * C + z];
+                    }
+                    Side<MODEL, D> sn;
+                    load_side<MODEL, D>(d, nb, dim, sn);
+                    const double zc = d.center[dim * C + z], nc = d.center[dim * C + nb], zs = d.size[z];
+                    const int zlev = d.level[z];
+                    if (kind == WK_OWN) {
+                        face_flux_sides_body<MODEL, D>(c, sz, zc, zs, zlev, sn, nc, dim, 0, dt, F);
+                        if ((q >> 11) & 1) {
+#pragma unroll
+                            for (int e = 0; e < NF; ++e) {
+                                GMEM double &a = d.acc[((2 * dim + 1) * NF + e) * C + nb];
+                                a = a - F[e];
+                            }
+                        }
+                    } else {
+                        face_flux_sides_body<MODEL, D>(c, sn, nc, zs, zlev, sz, zc, dim, 1, dt, F);
+                    }
+                }
+#pragma unroll
+                for (int e = 0; e < NF; ++e) M[dim][e][zl] = F[e];
+            }
         }
         __syncthreads();
         // + faces axis by axis; after each axis the cell adds its - and +
         // face in its face order
-        double ys = 0.0, yc = 0.0;
-        Side<MODEL, D> sy;
         double tot[NF];
 #pragma unroll
         for (int e = 0; e < NF; ++e) tot[e] = 0.0;
-        if (part) {
-            ys = d.size[y];
-#pragma unroll
-            for (int e = 0; e < NE; ++e) {
-                sy.p[e] = d.p[e * C + y];
-                sy.r[e] = d.rhs[e * C + y];
-            }
-            sy.t = d.t[y];
-            if constexpr (MODEL == MULTI) sy.kap = d.k[S::IKAPPA * C + y];
-            if constexpr (S::NG > 0) {
-#pragma unroll
-                for (int a = 0; a < D; ++a)
-#pragma unroll
-                    for (int e = 0; e < S::NG; ++e) sy.g[a][e] = d.grad[(a * S::NGX + e) * C + y];
-            }
-        }
 #pragma unroll 1
         for (int dim = 0; dim < D; ++dim) {
             const int fm = 2 * dim, fp = 2 * dim + 1;
-            const int32_t nbp = pick_i<2 * D>(nbs, fp);
-            double F[NF];
-#pragma unroll
-            for (int e = 0; e < NF; ++e) F[e] = 0.0;
-            if ((cp >> dim) & 1) {
+            double yc = 0.0;
+            if (valid) {
 #pragma unroll
                 for (int e = 0; e < NE; ++e) sy.s[e] = d.slope[(dim * NE + e) * C + y];
                 if constexpr (MODEL == MULTI) sy.kn = d.k[(S::IKN + dim) * C + y];
                 yc = d.center[dim * C + y];
+            }
+            const int32_t nbp = pick_i<2 * D>(nbs, fp);
+            const int64_t op64 = (int64_t) nbp - base;
+            const bool inrp = nbp != 0 && op64 >= 0 && op64 < 64;
+            // the + neighbour's side and centre from its lane (every lane takes part)
+            const int op = inrp ? (int) op64 : lane;
+            Side<MODEL, D> sn;
+            shfl_side<MODEL, D>(sy, op, sn);
+            double cn = __shfl(yc, op);
+            double F[NF];
+#pragma unroll
+            for (int e = 0; e < NF; ++e) F[e] = 0.0;
+            if ((cp >> dim) & 1) {
                 if (nbp == 0) {
                     boundary_flux<MODEL, D>(d, c, y, fp, dim, 1, dt, F);
                 } else {
-                    Side<MODEL, D> sn;
-                    load_side<MODEL, D>(d, nbp, dim, sn);
-                    face_flux_sides_body<MODEL, D>(c, sy, yc, ys, ylev, sn, d.center[dim * C + nbp], dim, 1, dt, F);
+                    if (!inrp) {
+                        load_side<MODEL, D>(d, nbp, dim, sn);
+                        cn = d.center[dim * C + nbp];
+                    }
+                    face_flux_sides_body<MODEL, D>(c, sy, yc, ys, ylev, sn, cn, dim, 1, dt, F);
                     if ((int) ((nclp >> (5 * fp)) & 31) - 1 != L) {
 #pragma unroll
                         for (int e = 0; e < NF; ++e) {
@@ -2329,7 +2384,14 @@ __device__ __forceinline__ int block_exclusive_scan_1024(int v, int *lds, int &t
 // Cluster-L lists of the batch, in node order (the order of the reference's
 // acti_clusters after a rebuild is irrelevant to the results; node order keeps
 // the sweeps coalesced): per pool tile a count, one scan per item, a scatter.
-template <int OUT>
+// a node's flags, tree level, ACTI level and cluster in one word
+__device__ __forceinline__ uint32_t node_meta(const Dev &d, int64_t i) {
+    return (uint32_t) d.flags[i] | (uint32_t) (uint8_t) d.level[i] << 8 | (uint32_t) (uint8_t) d.acti[i] << 16 |
+           (uint32_t) (uint8_t) d.cluster[i] << 24;
+}
+
+// META: also store every node's metadata word (d.meta) for the window flux op
+template <int OUT, bool META = false>
 __global__ void __launch_bounds__(256) k_sel_count(const Item *__restrict__ items, const int *__restrict__ toff,
                                                    int n_items, int *tilecnt) {
     __shared__ int wc[4];
@@ -2345,6 +2407,7 @@ __global__ void __launch_bounds__(256) k_sel_count(const Item *__restrict__ item
                 const int64_t id = base + k + threadIdx.x;
                 const bool pr = id >= 1 && id < it.n && it.d.cluster[id] == it.L;
                 cnt += __popcll(__ballot(pr));
+                if (META && id < it.n) it.d.meta[id] = node_meta(it.d, id);
             }
         }
         if ((threadIdx.x & 63) == 0) wc[threadIdx.x >> 6] = cnt;

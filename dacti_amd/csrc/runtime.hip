@@ -31,6 +31,7 @@ std::vector<Field> fields_of(dacti_handle *h, Dev &d) {
         {(void **) &d.error, 8, 1},          {(void **) &d.ctime, 8, 1},
         {(void **) &d.max_dt, 8, 1},         {(void **) &d.cfl, 8, 1},
         {(void **) &d.scratch, 8, 1},        {(void **) &d.brec, 4, 1},
+        {(void **) &d.meta, 4, 1},
     };
 }
 
