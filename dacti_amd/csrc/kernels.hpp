@@ -2182,8 +2182,10 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) k_
         bool member = false;
         int flv = 0;
         if (valid) {
-            member = d.cluster[y] == L;
-            flv = d.flags[y] | ((uint8_t) d.level[y]) << 8;
+            // the metadata word the dense list selection stored this substep
+            const uint32_t m = d.meta[y];
+            member = ((m >> 24) & 0xff) == (uint32_t) L;
+            flv = (int) (m & 0xffff);
 #pragma unroll
             for (int e = 0; e < NE; ++e) P[e][lane] = d.p[e * C + y];
             if constexpr (S::NG > 0) T[lane] = d.t[y];
@@ -2211,7 +2213,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) k_
 #pragma unroll
             for (int e = 0; e < NE; ++e) H[e][j] = d.p[e * C + nb];
             if constexpr (S::NG > 0) HT[j] = d.t[nb];
-            HFLV[j] = d.flags[nb] | ((uint8_t) d.level[nb]) << 8;
+            HFLV[j] = (int) (d.meta[nb] & 0xffff);
         }
         __syncthreads();
         if (member) {
@@ -2222,7 +2224,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) k_
                 const int32_t nb = nbs[f];
                 if (nb == 0) continue;
                 const int h = hs[f];
-                const int v = h < 0 ? FLV[nb - base] : h < HC ? HFLV[h] : (d.flags[nb] | ((uint8_t) d.level[nb]) << 8);
+                const int v = h < 0 ? FLV[nb - base] : h < HC ? HFLV[h] : (int) (d.meta[nb] & 0xffff);
                 if (!(v & F_LEAF) || ((v >> 8) & 0xff) != (flv >> 8)) fast = false;
             }
             if (!fast) {
@@ -2231,9 +2233,12 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) k_
                 double pc[NE], ctr[D], rhs[NE];
 #pragma unroll
                 for (int e = 0; e < NE; ++e) pc[e] = P[e][lane];
+                // the centre only serves the refinement-zone test of a cell
+                // that may still split
 #pragma unroll
-                for (int a = 0; a < D; ++a) ctr[a] = d.center[a * C + y];
-                const double ys = d.size[y], yt = d.t[y];
+                for (int a = 0; a < D; ++a) ctr[a] = (flv & F_CAN_SPLIT) ? d.center[a * C + y] : 0.0;
+                const double ys = d.size[y];
+                const double yt = S::NG > 0 ? T[lane] : 0.0;  // the time enters the gradients only
                 const double kap = MODEL == MULTI ? d.k[S::IKAPPA * C + y] : 0.0;
                 double a_ip_in, a_in_ip, a_in_ia;
                 rhs_coeffs<MODEL, D>(c, pc, kap, a_ip_in, a_in_ip, a_in_ia);
@@ -2247,9 +2252,10 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) k_
                 auto fetch = [&](int f, int dir, double *pv, double &tv, double &xv, bool &hv) {
                     const int32_t nb = pick_i<2 * D>(nbs, f);
                     if (nb == 0) {
-                        boundary_value_at<MODEL, D>(d, c, y, f, pc, yt, pv);
+                        const double tb = S::NG > 0 ? yt : d.t[y];
+                        boundary_value_at<MODEL, D>(d, c, y, f, pc, tb, pv);
                         xv = dir ? ys : -ys;
-                        tv = yt;
+                        tv = tb;
                         hv = false;
                         return;
                     }
