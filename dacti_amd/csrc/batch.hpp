@@ -35,7 +35,9 @@ struct BOps {
                 return (unsigned) std::min<int64_t>((w + 7) / 8 * 8, int64_t(1) << 22);
     }
 
-    static int launch_head(dacti_batch *b) {
+    static int launch_head(dacti_batch *b) { return launch_head_ops(b, true); }
+
+    static int launch_head_ops(dacti_batch *b, bool copy_status) {
         const hipStream_t st = b->stream;
         const unsigned gl = list_grid(b), gp = pool_grid(b);
         const int B = b->block, n = b->n;
@@ -72,6 +74,7 @@ struct BOps {
             b->err = std::string("head launch: ") + hipGetErrorString(e);
             return DACTI_E_HIP;
         }
+        if (!copy_status) return DACTI_OK;
         e = hipMemcpyAsync(b->h_st, b->d_st, sizeof(Status) * (size_t) n, hipMemcpyDeviceToHost, st);
         if (e != hipSuccess) {
             b->err = std::string("status copy: ") + hipGetErrorString(e);
@@ -102,7 +105,75 @@ struct BOps {
         return DACTI_OK;
     }
 
+    // A whole level substep with every decision on the device: head, the
+    // deferred sweep (acts only where cells were deferred), the remesh plan
+    // and phases (act only on instances with marks), tail.  Needs fixed
+    // grids (b->grid > 0) and pools with room for every listed cell to split
+    // (device_step checks); one wait at the end instead of two or three.
+    static int launch_step(dacti_batch *b) {
+        const hipStream_t st = b->stream;
+        const int n = b->n, B = b->block;
+        const unsigned G = (unsigned) b->grid;
+        Item *it = b->d_items;
+        int rc = launch_head_ops(b, false);
+        if (rc) return rc;
+        k_batch<<<G, B, 0, st>>>(it, b->d_mcs, b->d_toff, n, OpSweep<MODEL, D>{1});
+        k_clear_split_counts<true><<<(unsigned) ((n + 63) / 64), 64, 0, st>>>(it, n);
+        k_batch<<<G, B, 0, st>>>(it, b->d_mcs, b->d_toff, n, OpMark2<D, true>{});
+        k_remesh_plan<<<1, 64, 0, st>>>(it, n, S::NCH, B, b->d_toffr, b->d_toff2, b->d_toffp);
+        k_rn_count<<<G, 256, 0, st>>>(it, b->d_toffr, n, b->d_tilecnt);
+        k_sel_scan<2><<<1, 1024, 0, st>>>(it, b->d_toffr, n, b->d_tilecnt);
+        k_rn_scatter<<<G, 256, 0, st>>>(it, b->d_toffr, n, b->d_tilecnt);
+        k_batch<<<G, B, 0, st>>>(it, b->d_mcs, b->d_toff, n, OpSplitSlopes<MODEL, D>{});
+        k_batch<<<G, B, 0, st>>>(it, b->d_mcs, b->d_toff, n, OpSplit<MODEL, D>{});
+        k_batch<<<G, B, 0, st>>>(it, b->d_mcs, b->d_toff, n, OpUnsplit<MODEL, D>{});
+        k_pool_neighbors<D><<<G, 256, 0, st>>>(it, b->d_mcs, b->d_toffp, n);
+        k_sel_count<1><<<G, 256, 0, st>>>(it, b->d_toffp, n, b->d_tilecnt);
+        k_sel_scan<1><<<1, 1024, 0, st>>>(it, b->d_toffp, n, b->d_tilecnt);
+        k_sel_scatter<1><<<G, 256, 0, st>>>(it, b->d_toffp, n, b->d_tilecnt);
+        k_batch<<<G, B, 0, st>>>(it, b->d_mcs, b->d_toff2, n, OpReassign<MODEL, D>{});
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) {
+            b->err = std::string("remesh launch: ") + hipGetErrorString(e);
+            return DACTI_E_HIP;
+        }
+        return launch_tail(b);
+    }
+
+    // Whether a substep can run as launch_step: fixed grids and each
+    // participating pool with room for all its listed cells to split (grown
+    // here when that growth is small); the tile-count table sized for the
+    // pools and lists either way.
+    static bool device_step(dacti_batch *b, const std::vector<char> &part, int L, int &rc) {
+        rc = 0;
+        if (b->grid <= 0) return false;
+        int64_t tiles = 0;
+        for (int i = 0; i < b->n; ++i) {
+            dacti_handle *h = b->hs[(size_t) i];
+            if (!part[(size_t) i]) continue;
+            const int64_t grow = (int64_t) S::NCH * h->counts[L];
+            if (grow > (int64_t(1) << 22)) return false;
+            tiles += (h->n + grow + POOL_TILE - 1) / POOL_TILE + (h->counts[L] + 255) / 256 + 1;
+        }
+        for (int i = 0; i < b->n; ++i) {
+            if (!part[(size_t) i]) continue;
+            dacti_handle *h = b->hs[(size_t) i];
+            const int64_t need = h->n + (int64_t) S::NCH * h->counts[L];
+            if (need > h->d.C) {
+                Bind bind(b, i);
+                rc = ensure_capacity(h, need + need / 2);
+                if (rc) {
+                    fail_from(b, i);
+                    return false;
+                }
+            }
+        }
+        rc = batch_ensure_tiles(b, tiles + 64);
+        return rc == 0;
+    }
+
     // run a fixed sequence, from a graph captured on first use when graphs are on
+
     static int run_seq(dacti_batch *b, hipGraphExec_t *g, int (*seq)(dacti_batch *)) {
         if (!b->graphs || b->grid <= 0) return seq(b);
         if (!*g) {
@@ -305,6 +376,9 @@ struct BOps {
         b->h_toff[n] = (int) tl;
         int rc = 0;
         bool remeshed = false;
+        if (any_list && !(n == 1 && b->hs[0]->dense == 2) && device_step(b, part, L, rc))
+            return substep_device(b, L, part, changed, force_break);
+        if (rc) return rc;
         if (any_changed) {
             rc = upload_tables(b, changed);
             if (rc) return rc;
@@ -397,6 +471,61 @@ struct BOps {
             b->flux_ms += a;
             b->substep_ms += a + c + m;
         }
+        return finish_substep(b, L, part, changed);
+    }
+
+    // one substep as launch_step (the items were prepared by substep())
+    static int substep_device(dacti_batch *b, int L, std::vector<char> &part, std::vector<char> &changed,
+                              int *force_break) {
+        const int n = b->n;
+        int rc = upload_tables(b, changed);
+        if (rc) return rc;
+        b->dense = false;
+        if (b->timing && hipEventRecord(b->ev[0], b->stream) != hipSuccess) return DACTI_E_HIP;
+        rc = run_seq(b, &b->g_step, launch_step);
+        if (rc) return rc;
+        if (b->timing && hipEventRecord(b->ev[1], b->stream) != hipSuccess) return DACTI_E_HIP;
+        rc = sync(b);
+        if (rc) return rc;
+        if (b->timing) {
+            float a = 0;
+            if (hipEventElapsedTime(&a, b->ev[0], b->ev[1]) != hipSuccess) return DACTI_E_HIP;
+            b->flux_ms += a;
+            b->substep_ms += a;
+        }
+        b->launches++;
+        for (int i = 0; i < n; ++i) {
+            Item &it = b->h_items[i];
+            if (it.nl == 0) continue;
+            dacti_handle *h = b->hs[(size_t) i];
+            const Status &s = b->h_st[i];
+            h->flux_launches++;
+            h->cell_updates += it.nl;
+            b->cell_updates += it.nl;
+            if (s.n_list != it.nl) {
+                b->err = "internal: cluster list size differs from the level histogram";
+                return DACTI_E_STATE;
+            }
+            if (s.nan_flag) {
+                // this instance stops here (the device skipped its sweep,
+                // remesh and LTS apply, and its level histogram)
+                *force_break = 1;
+                b->broken[(size_t) i] = 1;
+                part[(size_t) i] = 0;
+                changed[(size_t) i] = 0;
+                it.nl = 0;
+                it.flush = 0;
+                continue;
+            }
+            h->n = s.n_pool;
+        }
+        return finish_substep(b, L, part, changed);
+    }
+
+    // the host side of a substep's end: cluster sizes, counters, LTS times,
+    // the caller's hook
+    static int finish_substep(dacti_batch *b, int L, std::vector<char> &part, std::vector<char> &changed) {
+        const int n = b->n;
         for (int i = 0; i < n; ++i) {
             dacti_handle *h = b->hs[(size_t) i];
             if (changed[(size_t) i])
@@ -432,6 +561,7 @@ struct BOps {
 
     // highest level any instance may run: a level with a cluster, a pending
     // buffer, or at most the instance's max_acti_level
+
     static void update_top(dacti_batch *b) {
         int top = 0;
         for (int i = 0; i < b->n; ++i) {

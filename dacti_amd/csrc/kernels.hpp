@@ -61,6 +61,7 @@ struct Status {
     int32_t raised[16];
     int32_t n_list;         // size of the substep's cluster list as selected on the device
     int32_t n_list2;        // size of the cluster list rebuilt after a remesh
+    int64_t n_pool;         // pool size after this substep's splits (substeps planned on the device)
 };
 
 __device__ __forceinline__ bool is_leaf(const Dev &d, int32_t i) { return d.flags[i] & F_LEAF; }
@@ -2357,6 +2358,7 @@ struct OpSmoothCurvature {
 };
 struct OpApply {
     __device__ __forceinline__ void operator()(const Item &it, const MConst &c, int li) const {
+        if (it.st->nan_flag) return;  // an instance with a NaN state stops before the LTS apply
         apply_body(it.d, it.list, it.nl, li, it.L);
     }
 };
@@ -2710,19 +2712,59 @@ static __global__ void __launch_bounds__(256) k_pool_hist(const Item *__restrict
 // Launched with the substep's head for every instance that has marks (the
 // sweep counted them), so the remesh needs no wait of its own; run again,
 // after clearing its counts, when deferred cells received marks later.
-template <int D>
+// AGAIN: the repeat after the deferred sweep, which the substeps planned
+// on the device launch unconditionally (it acts only where cells were
+// deferred)
+template <int D, bool AGAIN = false>
 struct OpMark2 {
     __device__ __forceinline__ void operator()(const Item &it, const MConst &, int li) const {
         if (it.st->n_marked == 0 || it.st->nan_flag) return;
+        if (AGAIN && it.st->n_deferred == 0) return;
         mark2_body<D>(it.d, it.list, it.nl, li, it.L, it.rn, it.st);
     }
 };
 
-static __global__ void k_clear_split_counts(const Item *__restrict__ items, int n_items) {
+template <bool AGAIN = false>
+__global__ void k_clear_split_counts(const Item *__restrict__ items, int n_items) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n_items) return;
+    if (AGAIN && items[i].st->n_deferred == 0) return;
     items[i].st->n_ready_split = 0;
     items[i].st->n_new_blocks = 0;
+}
+
+// The remesh plan of a substep, on the device (what the host computes
+// between the head and the remesh otherwise): which instances remesh, their
+// pool size before and after the splits (the host guaranteed the capacity),
+// and the tile tables of the remesh phases (toffr: list tiles of 256,
+// toff2: the rebuilt lists in tiles of `block`) and of the grown pools
+// (toffp).  One thread; instances in order.
+static __global__ void k_remesh_plan(Item *__restrict__ items, int n_items, int nch, int block,
+                                     int *__restrict__ toffr, int *__restrict__ toff2, int *__restrict__ toffp) {
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    int tr = 0, t2 = 0, tp = 0;
+    for (int i = 0; i < n_items; ++i) {
+        Item &it = items[i];
+        Status *st = it.st;
+        toffr[i] = tr;
+        toff2[i] = t2;
+        toffp[i] = tp;
+        const bool nan = st->nan_flag != 0;
+        const bool mesh = it.nl > 0 && st->n_marked > 0 && !nan;
+        it.meshing = mesh ? 1 : 0;
+        if (mesh) {
+            tr += (it.nl + 255) / 256;
+            it.n_before = it.n;
+            it.n = it.n + (int64_t) nch * st->n_new_blocks;
+            t2 += (int) ((2 * (int64_t) it.nl + (int64_t) nch * st->n_ready_split + block - 1) / block);
+        }
+        st->n_pool = it.n;
+        // an instance that broke down takes no part in the tail's pool passes
+        if ((it.nl > 0 || it.flush) && !nan) tp += (int) ((it.n + POOL_TILE - 1) / POOL_TILE);
+    }
+    toffr[n_items] = tr;
+    toff2[n_items] = t2;
+    toffp[n_items] = tp;
 }
 
 // exclusive scan of the new-block flags over each item's list: per list

@@ -245,7 +245,8 @@ void batch_drop_graphs(dacti_batch *b) {
     if (b->g_head) (void) hipGraphExecDestroy(b->g_head);
     if (b->g_tail) (void) hipGraphExecDestroy(b->g_tail);
     if (b->g_head_dense) (void) hipGraphExecDestroy(b->g_head_dense);
-    b->g_head = b->g_tail = b->g_head_dense = nullptr;
+    if (b->g_step) (void) hipGraphExecDestroy(b->g_step);
+    b->g_head = b->g_tail = b->g_head_dense = b->g_step = nullptr;
 }
 
 void batch_free(dacti_batch *b) {

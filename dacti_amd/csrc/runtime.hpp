@@ -124,6 +124,7 @@ struct dacti_batch {
     bool graphs = false;
     hipGraphExec_t g_head = nullptr, g_tail = nullptr;
     hipGraphExec_t g_head_dense = nullptr;  // head with the window flux op (a dense cluster of a batch of one)
+    hipGraphExec_t g_step = nullptr;        // a whole substep planned on the device
     bool dense = false;                     // this substep takes the window flux op
     bool timing = false;
     // head start/end, tail start/end, remesh start/end
