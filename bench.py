@@ -396,8 +396,9 @@ def bubble(args):
     tree = scene.build_tree(case)
     print(f"[bench] shock-bubble tree: {tree.n} nodes, {len(tree.leaves())} leaves", file=sys.stderr, flush=True)
     g = Integrator(case, tree)
-    if args.graphs:
-        g.set_launch(block=128, grid=2048, graphs=True)
+    if args.graphs or args.grid:
+        # a fixed grid lets each substep run planned on the device (one wait)
+        g.set_launch(block=args.block, grid=args.grid or 2048, graphs=bool(args.graphs))
     for w in range(args.warmup):
         g.integrate()
     g.set_timing(True)
@@ -434,7 +435,9 @@ def bubble(args):
         "config": {"workload": f"2D shock-bubble (five-equation model), 64x32 base + {args.extra} levels, "
                                "remesh every substep, LTS on",
                    "nodes": s1.n_nodes, "active_leaves": s1.n_active, "max_acti_level": s1.max_acti_level,
-                   "cell_updates_timed": updates, "graphs": bool(args.graphs)},
+                   "cell_updates_timed": updates, "graphs": bool(args.graphs),
+                   "launch": {"block": args.block if (args.graphs or args.grid) else 256,
+                              "grid": (args.grid or 2048) if (args.graphs or args.grid) else 0}},
         "roofline": substep_roofline(B_ALG_MULTI2, updates, s1.flux_launches - s0.flux_launches,
                                      s1.substep_kernel_ms - s0.substep_kernel_ms,
                                      "one level substep (head, remesh and tail sequences)"),
@@ -513,6 +516,8 @@ def main():
                          "learned: refinement predictor vs the estimator; drop: 3D shock-drop with AMR; "
                          "bubble: 2D shock-bubble with AMR")
     ap.add_argument("--graphs", action="store_true", help="bubble: persistent grid and graph replay")
+    ap.add_argument("--grid", type=int, default=0, help="bubble: persistent grid (0: one block per tile)")
+    ap.add_argument("--block", type=int, default=128, help="bubble: threads per list tile with --grid/--graphs")
     ap.add_argument("--dense", type=int, default=1, choices=(0, 1, 2),
                     help="box: flux op by windows for dense clusters (0 never, 1 auto, 2 always)")
     ap.add_argument("--base", type=int, default=5)

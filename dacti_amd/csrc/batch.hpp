@@ -93,11 +93,8 @@ struct BOps {
         k_zero_hist<<<(unsigned) n, 64, 0, st>>>(it, n);
         k_pool_hist<<<std::min(gp, 1024u), 256, 0, st>>>(it, b->d_toffp, n);
         hipError_t e = hipGetLastError();
-        if (e == hipSuccess)
-            e = hipMemcpyAsync(b->h_hist, b->d_hist, sizeof(unsigned long long) * 16 * (size_t) n,
-                               hipMemcpyDeviceToHost, st);
-        if (e == hipSuccess)
-            e = hipMemcpyAsync(b->h_st, b->d_st, sizeof(Status) * (size_t) n, hipMemcpyDeviceToHost, st);
+        // status and level histograms: one block, one copy
+        if (e == hipSuccess) e = hipMemcpyAsync(b->h_down, b->d_down, b->down_bytes, hipMemcpyDeviceToHost, st);
         if (e != hipSuccess) {
             b->err = std::string("tail launch: ") + hipGetErrorString(e);
             return DACTI_E_HIP;
@@ -233,17 +230,8 @@ struct BOps {
         b->h_toffp[b->n] = (int) tp;
         int rc = batch_ensure_tiles(b, tp);
         if (rc) return rc;
-        const size_t n1 = (size_t) b->n + 1;
-        hipError_t e = hipMemcpyAsync(b->d_items, b->h_items, sizeof(Item) * (size_t) b->n, hipMemcpyHostToDevice,
-                                      b->stream);
-        if (e == hipSuccess)
-            e = hipMemcpyAsync(b->d_toff, b->h_toff, sizeof(int) * n1, hipMemcpyHostToDevice, b->stream);
-        if (e == hipSuccess)
-            e = hipMemcpyAsync(b->d_toffp, b->h_toffp, sizeof(int) * n1, hipMemcpyHostToDevice, b->stream);
-        if (e == hipSuccess)
-            e = hipMemcpyAsync(b->d_toffr, b->h_toffr, sizeof(int) * n1, hipMemcpyHostToDevice, b->stream);
-        if (e == hipSuccess)
-            e = hipMemcpyAsync(b->d_toff2, b->h_toff2, sizeof(int) * n1, hipMemcpyHostToDevice, b->stream);
+        // items and the four tile tables: one block, one copy
+        const hipError_t e = hipMemcpyAsync(b->d_up, b->h_up, b->up_bytes, hipMemcpyHostToDevice, b->stream);
         if (e != hipSuccess) {
             b->err = std::string("table upload: ") + hipGetErrorString(e);
             return DACTI_E_HIP;

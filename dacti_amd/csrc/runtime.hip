@@ -253,11 +253,9 @@ void batch_free(dacti_batch *b) {
     if (!b) return;
     if (b->stream) (void) hipStreamSynchronize(b->stream);
     batch_drop_graphs(b);
-    for (void *p : {(void *) b->d_items, (void *) b->d_mcs, (void *) b->d_toff, (void *) b->d_toffp, (void *) b->d_toffr,
-                    (void *) b->d_toff2, (void *) b->d_tilecnt, (void *) b->d_st, (void *) b->d_hist})
+    for (void *p : {b->d_up, (void *) b->d_mcs, (void *) b->d_tilecnt, b->d_down})
         if (p) (void) hipFree(p);
-    for (void *p : {(void *) b->h_items, (void *) b->h_toff, (void *) b->h_toffp, (void *) b->h_toffr, (void *) b->h_toff2,
-                    (void *) b->h_st, (void *) b->h_hist})
+    for (void *p : {b->h_up, b->h_down})
         if (p) (void) hipHostFree(p);
     for (auto &e : b->ev)
         if (e) (void) hipEventDestroy(e);
@@ -306,21 +304,37 @@ int batch_make(dacti_handle *const *hs, int n, hipStream_t stream, dacti_batch *
         b->own_stream = true;
     }
     const size_t n1 = (size_t) n + 1;
-    bool ok = hipMalloc(&b->d_items, sizeof(Item) * (size_t) n) == hipSuccess &&
-              hipMalloc(&b->d_mcs, sizeof(MConst) * (size_t) n) == hipSuccess &&
-              hipMalloc(&b->d_toff, sizeof(int) * n1) == hipSuccess &&
-              hipMalloc(&b->d_toffp, sizeof(int) * n1) == hipSuccess &&
-              hipMalloc(&b->d_toffr, sizeof(int) * n1) == hipSuccess &&
-              hipMalloc(&b->d_toff2, sizeof(int) * n1) == hipSuccess &&
-              hipHostMalloc(&b->h_toffr, sizeof(int) * n1) == hipSuccess &&
-              hipHostMalloc(&b->h_toff2, sizeof(int) * n1) == hipSuccess &&
-              hipMalloc(&b->d_st, sizeof(Status) * (size_t) n) == hipSuccess &&
-              hipMalloc(&b->d_hist, sizeof(unsigned long long) * 16 * (size_t) n) == hipSuccess &&
-              hipHostMalloc(&b->h_items, sizeof(Item) * (size_t) n) == hipSuccess &&
-              hipHostMalloc(&b->h_toff, sizeof(int) * n1) == hipSuccess &&
-              hipHostMalloc(&b->h_toffp, sizeof(int) * n1) == hipSuccess &&
-              hipHostMalloc(&b->h_st, sizeof(Status) * (size_t) n) == hipSuccess &&
-              hipHostMalloc(&b->h_hist, sizeof(unsigned long long) * 16 * (size_t) n) == hipSuccess;
+    // the per-substep uploads (items and four tile tables) and downloads
+    // (status and level histograms) each sit in one block with the same
+    // layout on both sides, so each direction is one copy
+    auto al = [](size_t x) { return (x + 255) & ~size_t(255); };
+    const size_t o_toff = al(sizeof(Item) * (size_t) n), o_toffp = o_toff + al(sizeof(int) * n1),
+                 o_toffr = o_toffp + al(sizeof(int) * n1), o_toff2 = o_toffr + al(sizeof(int) * n1);
+    b->up_bytes = o_toff2 + sizeof(int) * n1;
+    const size_t o_hist = al(sizeof(Status) * (size_t) n);
+    b->down_bytes = o_hist + sizeof(unsigned long long) * 16 * (size_t) n;
+    bool ok = hipMalloc(&b->d_up, b->up_bytes) == hipSuccess && hipHostMalloc(&b->h_up, b->up_bytes) == hipSuccess &&
+              hipMalloc(&b->d_down, b->down_bytes) == hipSuccess &&
+              hipHostMalloc(&b->h_down, b->down_bytes) == hipSuccess &&
+              hipMalloc(&b->d_mcs, sizeof(MConst) * (size_t) n) == hipSuccess;
+    if (ok) {
+        char *du = (char *) b->d_up, *hu = (char *) b->h_up, *dd = (char *) b->d_down, *hd = (char *) b->h_down;
+        b->d_items = (Item *) du;
+        b->h_items = (Item *) hu;
+        b->d_toff = (int *) (du + o_toff);
+        b->h_toff = (int *) (hu + o_toff);
+        b->d_toffp = (int *) (du + o_toffp);
+        b->h_toffp = (int *) (hu + o_toffp);
+        b->d_toffr = (int *) (du + o_toffr);
+        b->h_toffr = (int *) (hu + o_toffr);
+        b->d_toff2 = (int *) (du + o_toff2);
+        b->h_toff2 = (int *) (hu + o_toff2);
+        b->d_st = (Status *) dd;
+        b->h_st = (Status *) hd;
+        b->d_hist = (unsigned long long *) (dd + o_hist);
+        b->h_hist = (unsigned long long *) (hd + o_hist);
+        std::memset(b->h_up, 0, b->up_bytes);
+    }
     for (auto &e : b->ev) ok = ok && hipEventCreate(&e) == hipSuccess;
     if (!ok) {
         err = "batch allocation failed";

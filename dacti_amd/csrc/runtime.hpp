@@ -118,6 +118,10 @@ struct dacti_batch {
     int64_t tilecnt_cap = 0;
     Status *h_st = nullptr, *d_st = nullptr;
     unsigned long long *h_hist = nullptr, *d_hist = nullptr;
+    // the blocks the pointers above live in: items and tile tables (one
+    // upload per substep), status and histograms (one download)
+    void *d_up = nullptr, *h_up = nullptr, *d_down = nullptr, *h_down = nullptr;
+    size_t up_bytes = 0, down_bytes = 0;
     // launch configuration: threads per list tile, persistent grid (0 = one
     // block per tile), HIP graphs for the fixed kernel sequences
     int block = 256, grid = 0;
