@@ -272,14 +272,20 @@ __device__ __forceinline__ void wave_count_add(bool pred, int32_t *counter) {
     if (m && (threadIdx.x & 63) == (unsigned) (__ffsll((long long) m) - 1)) atomicAdd(counter, (int32_t) __popcll(m));
 }
 
-// Cell data one side of a face needs (integrator/cell.hpp:8-38)
+// Cell data one side of a face needs (integrator/cell.hpp:8-38).  The
+// gradient (NS, multi) enters only the viscous and heat terms at the end of
+// the face flux, so a side carries where it lives (the node's first
+// gradient component and the component stride) and the flux reads it there:
+// registers are not held for it through the Riemann solve.
 template <int MODEL, int D>
 struct Side {
     using S = Sz<MODEL, D>;
     double p[S::NE], s[S::NE], r[S::NE];
     double t;
     double kap, kn;               // multi: k[kappa], k[ikn + dim]
-    double g[D][S::NGX];          // NS / multi: gradient
+    const GMEM double *g;         // NS / multi: &grad[node]; component (a, e) at g[(a * NGX + e) * gC]
+    int64_t gC;
+    __device__ __forceinline__ double grad(int a, int e) const { return g[(a * S::NGX + e) * gC]; }
 };
 
 template <int MODEL, int D>
@@ -298,10 +304,8 @@ __device__ __forceinline__ void load_side(const Dev &d, int32_t i, int dim, Side
         s.kn = d.k[(S::IKN + dim) * C + i];
     }
     if constexpr (S::NG > 0) {
-#pragma unroll
-        for (int a = 0; a < D; ++a)
-#pragma unroll
-            for (int e = 0; e < S::NG; ++e) s.g[a][e] = d.grad[(a * S::NGX + e) * C + i];
+        s.g = d.grad + i;
+        s.gC = C;
     }
 }
 
@@ -355,7 +359,7 @@ __device__ __forceinline__ void riemann_flux(const MConst &c, const double *pn, 
         }
         double du[D][S::NGX], vel[D], fv[D + 1];
         for (int a = 0; a < D; ++a)
-            for (int e = 0; e < S::NG; ++e) du[a][e] = 0.5 * (n.g[a][e] + pz.g[a][e]);
+            for (int e = 0; e < S::NG; ++e) du[a][e] = 0.5 * (n.grad(a, e) + pz.grad(a, e));
         for (int i = 0; i < D; ++i) vel[i] = 0.5 * (n.p[IU + i] + pz.p[IU + i]);
         visc_Fv<D, S::NGX>(du, vel, model_mu<MODEL, D>(c, pm), dim, fv);
         for (int i = 0; i <= D; ++i) F[IU + i] -= fv[i];
@@ -370,7 +374,7 @@ __device__ __forceinline__ void riemann_flux(const MConst &c, const double *pn, 
         if constexpr (MODEL == NAVIER_STOKES) {
             double du[D][S::NGX], vel[D], fv[D + 1];
             for (int a = 0; a < D; ++a)
-                for (int e = 0; e < S::NG; ++e) du[a][e] = 0.5 * (n.g[a][e] + pz.g[a][e]);
+                for (int e = 0; e < S::NG; ++e) du[a][e] = 0.5 * (n.grad(a, e) + pz.grad(a, e));
             for (int i = 0; i < D; ++i) vel[i] = 0.5 * (n.p[IU + i] + pz.p[IU + i]);
             visc_Fv<D, S::NGX>(du, vel, c.ns_mu, dim, fv);
             for (int i = 0; i <= D; ++i) out[IU + i] -= fv[i] * area * (t1 - t0);
@@ -529,10 +533,8 @@ __device__ void flux_cell(const Dev &d, const MConst &c, int32_t y, int L, doubl
     sy.t = d.t[y];
     if constexpr (MODEL == MULTI) sy.kap = d.k[S::IKAPPA * C + y];
     if constexpr (S::NG > 0) {
-#pragma unroll
-        for (int a = 0; a < D; ++a)
-#pragma unroll
-            for (int e = 0; e < S::NG; ++e) sy.g[a][e] = d.grad[(a * S::NGX + e) * C + y];
+        sy.g = d.grad + y;
+        sy.gC = C;
     }
     double tot[NF];
 #pragma unroll
@@ -712,10 +714,8 @@ __device__ __forceinline__ void flux_simple_body(const Dev &d, const MConst &c, 
     sy.t = d.t[y];
     if constexpr (MODEL == MULTI) sy.kap = d.k[S::IKAPPA * C + y];
     if constexpr (S::NG > 0) {
-#pragma unroll
-        for (int a = 0; a < D; ++a)
-#pragma unroll
-            for (int e = 0; e < S::NG; ++e) sy.g[a][e] = d.grad[(a * S::NGX + e) * C + y];
+        sy.g = d.grad + y;
+        sy.gC = C;
     }
     double tot[NF];
 #pragma unroll
@@ -1803,6 +1803,7 @@ __global__ void __launch_bounds__(256) k_batch(const Item *__restrict__ items, c
     }
 }
 
+
 template <int MODEL, int D>
 struct OpFluxSimple {
     __device__ __forceinline__ void operator()(const Item &it, const MConst &c, int li) const {
@@ -1864,10 +1865,8 @@ __device__ __forceinline__ void shfl_side(const Side<MODEL, D> &a, int o, Side<M
         b.kn = __shfl(a.kn, o);
     }
     if constexpr (S::NG > 0) {
-#pragma unroll
-        for (int x = 0; x < D; ++x)
-#pragma unroll
-            for (int e = 0; e < S::NG; ++e) b.g[x][e] = __shfl(a.g[x][e], o);
+        b.g = (const GMEM double *) (uintptr_t) __shfl((unsigned long long) (uintptr_t) a.g, o);
+        b.gC = a.gC;
     }
 }
 
@@ -1917,10 +1916,8 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) k_
             sy.t = d.t[y];
             if constexpr (MODEL == MULTI) sy.kap = d.k[S::IKAPPA * C + y];
             if constexpr (S::NG > 0) {
-#pragma unroll
-                for (int a = 0; a < D; ++a)
-#pragma unroll
-                    for (int e = 0; e < S::NG; ++e) sy.g[a][e] = d.grad[(a * S::NGX + e) * C + y];
+                sy.g = d.grad + y;
+                sy.gC = C;
             }
         }
         int32_t nbs[2 * D];
@@ -2016,10 +2013,8 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) k_
                         sz.kn = d.k[(S::IKN + dim) * C + z];
                     }
                     if constexpr (S::NG > 0) {
-#pragma unroll
-                        for (int a = 0; a < D; ++a)
-#pragma unroll
-                            for (int e = 0; e < S::NG; ++e) sz.g[a][e] = d.grad[(a * S::NGX + e) * C + z];
+                        sz.g = d.grad + z;
+                        sz.gC = C;
                     }
                     Side<MODEL, D> sn;
                     load_side<MODEL, D>(d, nb, dim, sn);
