@@ -38,7 +38,9 @@ inline int pad16(int x) { return (x + 15) / 16 * 16; }
 
 // C(m, n) = sum_k A(m, k) B(k, n), A(m, k) = A[m*sam + k*sak], B(k, n) =
 // B[k*sbk + n*sbn] (BF16 bits); M, N, K multiples of 16.  Epilogue: + bias[n]
-// (if given), ReLU (if relu), FP32 store to C[m*ldc + n], BF16 copy to Cb.
+// (if given), ReLU (if relu), zero where mask[m*ldc + n] <= 0 (if mask: the
+// derivative of the ReLU whose output is mask), FP32 store to C[m*ldc + n],
+// BF16 copy to Cb.
 // Lane layout of the 16x16x16 BF16 MFMA: lane l holds A(l%16, 4(l/16)+j) and
 // B(4(l/16)+j, l%16) for j < 4, and returns C(4(l/16)+i, l%16) for i < 4.
 // With ks > 1 the reduction is split into ks equal ranges (each a multiple
@@ -47,7 +49,8 @@ inline int pad16(int x) { return (x + 15) / 16 * 16; }
 __global__ void __launch_bounds__(256) k_gemm(const uint16_t *__restrict__ A, int64_t sam, int64_t sak,
                                               const uint16_t *__restrict__ B, int64_t sbk, int64_t sbn, int M, int N,
                                               int K, float *__restrict__ C, int64_t ldc, uint16_t *__restrict__ Cb,
-                                              const float *__restrict__ bias, int relu, int ks) {
+                                              const float *__restrict__ bias, int relu, int ks,
+                                              const float *__restrict__ mask) {
     const int wave = (int) ((blockIdx.x * (int64_t) blockDim.x + threadIdx.x) >> 6);
     const int lane = threadIdx.x & 63;
     const int tn = N / 16, tiles = (M / 16) * tn;
@@ -75,6 +78,7 @@ __global__ void __launch_bounds__(256) k_gemm(const uint16_t *__restrict__ A, in
         float v = c[i];
         if (bias) v += bias[n];
         if (relu) v = v > 0.0f ? v : 0.0f;
+        if (mask) v = mask[(int64_t) m * ldc + n] > 0.0f ? v : 0.0f;
         C[(int64_t) m * ldc + n] = v;
         if (Cb) Cb[(int64_t) m * ldc + n] = to_bf16(v);
     }
@@ -94,12 +98,21 @@ __global__ void k_to_bf16(const float *__restrict__ x, uint16_t *__restrict__ y,
     if (i < n) y[i] = to_bf16(x[i]);
 }
 
-// rows of the padded BF16 input: row r <- sample idx[r] (r < nb), zero beyond
+// training steps replayed from a graph find their batch here: batch b of
+// the epoch takes permutation entries [b * batch_rows, ...) and the Adam
+// corrections c12[2b], c12[2b + 1]
+struct TrainCtl {
+    int64_t batch;
+};
+
+// rows of the padded BF16 input: row r <- sample idx[r] (r < nb), zero
+// beyond; with ctl, idx is advanced by ctl->batch * nb
 __global__ void k_gather(const float *__restrict__ x, int64_t ldx, int nin, const int32_t *__restrict__ idx, int nb,
                          int rows, uint16_t *__restrict__ xb, int ldb, const int32_t *__restrict__ y,
-                         int32_t *__restrict__ yb) {
+                         int32_t *__restrict__ yb, const TrainCtl *__restrict__ ctl) {
     const int r = blockIdx.x, c = threadIdx.x;
     if (r >= rows) return;
+    if (ctl) idx += ctl->batch * nb;
     const int64_t s = r < nb ? idx[r] : -1;
     for (int k = c; k < ldb; k += blockDim.x) xb[(int64_t) r * ldb + k] = (s >= 0 && k < nin) ? to_bf16(x[s * ldx + k]) : 0;
     if (c == 0) yb[r] = (s >= 0 && y) ? y[s] : -1;
@@ -158,33 +171,54 @@ __global__ void __launch_bounds__(1024) k_colsum(const float *__restrict__ dz, i
     }
 }
 
-// dZ_prev = dA * (post-activation > 0), FP32 and BF16
-__global__ void k_relu_back(const float *__restrict__ da, const float *__restrict__ act, int64_t n,
-                            float *__restrict__ dz, uint16_t *__restrict__ dz16) {
+// one parameter tensor of the Adam step (flat elements [off, off + n))
+struct AdamT {
+    float *w, *g, *m, *v;
+    uint16_t *w16;  // BF16 copy for the next forward product (weights only)
+    int64_t n, off;
+};
+
+// Adam over every weight and bias tensor in one launch, refreshing the BF16
+// weight copies; the bias corrections c1, c2 come from the host, or with
+// ctl from c12 at the current batch
+__global__ void k_adam_all(const AdamT *__restrict__ tab, int nt, int64_t total, float lr, float c1, float c2,
+                           const TrainCtl *__restrict__ ctl, const float *__restrict__ c12) {
     const int64_t i = blockIdx.x * (int64_t) blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const float g = act[i] > 0.0f ? da[i] : 0.0f;
-    dz[i] = g;
-    dz16[i] = to_bf16(g);
+    if (i >= total) return;
+    int lo = 0, hi = nt - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (tab[mid].off <= i) lo = mid;
+        else hi = mid - 1;
+    }
+    const AdamT &t = tab[lo];
+    const int64_t j = i - t.off;
+    if (ctl) {
+        c1 = c12[2 * ctl->batch];
+        c2 = c12[2 * ctl->batch + 1];
+    }
+    const float gi = t.g[j];
+    const float mi = 0.9f * t.m[j] + 0.1f * gi;
+    const float vi = 0.999f * t.v[j] + 0.001f * gi * gi;
+    t.m[j] = mi;
+    t.v[j] = vi;
+    const float w = t.w[j] - lr * (mi / c1) / (sqrtf(vi / c2) + 1e-8f);
+    t.w[j] = w;
+    if (t.w16) t.w16[j] = to_bf16(w);
 }
 
-__global__ void k_adam(float *__restrict__ w, const float *__restrict__ g, float *__restrict__ m,
-                       float *__restrict__ v, int64_t n, float lr, float c1, float c2) {
-    const int64_t i = blockIdx.x * (int64_t) blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const float gi = g[i];
-    const float mi = 0.9f * m[i] + 0.1f * gi;
-    const float vi = 0.999f * v[i] + 0.001f * gi * gi;
-    m[i] = mi;
-    v[i] = vi;
-    w[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + 1e-8f);
-}
-
-// per-sample losses of a batch scattered to their sample slots
+// per-sample losses of a batch scattered to their sample slots (with ctl:
+// the current batch's slots)
 __global__ void k_scatter_loss(const float *__restrict__ lb, const int32_t *__restrict__ idx, int nb,
-                               float *__restrict__ all) {
+                               float *__restrict__ all, const TrainCtl *__restrict__ ctl) {
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (ctl) idx += ctl->batch * nb;
     if (r < nb) all[idx[r]] = lb[r];
+}
+
+// the last kernel of a replayed training step
+__global__ void k_next_batch(TrainCtl *ctl) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) ctl->batch += 1;
 }
 
 inline unsigned nblk(int64_t n, int b = 256) { return (unsigned) ((n + b - 1) / b); }
@@ -206,7 +240,17 @@ struct dacti_mlp {
     float *part = nullptr;  // split-reduction partial tiles
     int64_t part_cap = 0;
     uint16_t *dZ16 = nullptr;
+    // the next layer's dZ while the current one is read (the masked input
+    // gradient is written by the product that reads dZ)
+    float *dZb = nullptr;
+    uint16_t *dZ16b = nullptr;
     int32_t *yb = nullptr;
+    AdamT *d_adam = nullptr;  // every parameter tensor, for k_adam_all
+    int n_adam = 0;
+    int64_t adam_total = 0;
+    TrainCtl *d_ctl = nullptr;
+    float *d_c12 = nullptr;  // Adam corrections of one epoch's replayed steps
+    int64_t c12_cap = 0;
     hipStream_t s = nullptr;
     int dev = 0;  // device the network lives on
     hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -241,7 +285,7 @@ int ensure_rows(dacti_mlp *m, int rows) {
     auto fr = [](void *p) {
         if (p) (void) hipFree(p);
     };
-    fr(m->X16), fr(m->dZ), fr(m->dA), fr(m->dZ16), fr(m->yb), fr(m->lossb);
+    fr(m->X16), fr(m->dZ), fr(m->dA), fr(m->dZ16), fr(m->yb), fr(m->lossb), fr(m->dZb), fr(m->dZ16b);
     for (auto p : m->Z) fr(p);
     for (auto p : m->A16) fr(p);
     const int cap = pad16(rows);
@@ -250,6 +294,7 @@ int ensure_rows(dacti_mlp *m, int rows) {
     int rc;
     if ((rc = dalloc(m, &m->X16, (size_t) cap * m->ip[0])) || (rc = dalloc(m, &m->dZ, (size_t) cap * maxw)) ||
         (rc = dalloc(m, &m->dA, (size_t) cap * maxw)) || (rc = dalloc(m, &m->dZ16, (size_t) cap * maxw)) ||
+        (rc = dalloc(m, &m->dZb, (size_t) cap * maxw)) || (rc = dalloc(m, &m->dZ16b, (size_t) cap * maxw)) ||
         (rc = dalloc(m, &m->yb, (size_t) cap)) ||
         (rc = dalloc(m, &m->lossb, (size_t) cap)))
         return rc;
@@ -265,10 +310,10 @@ int ensure_rows(dacti_mlp *m, int rows) {
 // A long reduction over few output tiles (the weight gradients: K = batch
 // rows) is split so that enough wavefronts run; otherwise one wave per tile.
 int gemm(dacti_mlp *m, const uint16_t *A, int64_t sam, int64_t sak, const uint16_t *B, int64_t sbk, int64_t sbn, int M,
-         int N, int K, float *C, int64_t ldc, uint16_t *Cb, const float *bias, int relu) {
+         int N, int K, float *C, int64_t ldc, uint16_t *Cb, const float *bias, int relu, const float *mask = nullptr) {
     const int64_t tiles = (int64_t) (M / 16) * (N / 16);
     int ks = 1;
-    if (!bias && !relu && !Cb)
+    if (!bias && !relu && !Cb && !mask)
         while (ks < 32 && tiles * ks < 2048 && K / (2 * ks) >= 64 && (K / 16) % (2 * ks) == 0) ks *= 2;
     const int64_t need = (int64_t) ks * M * ldc;
     if (ks > 1 && need > m->part_cap) {
@@ -279,7 +324,7 @@ int gemm(dacti_mlp *m, const uint16_t *A, int64_t sam, int64_t sak, const uint16
         m->part_cap = need;
     }
     k_gemm<<<nblk(tiles * ks * 64), 256, 0, m->s>>>(A, sam, sak, B, sbk, sbn, M, N, K, ks > 1 ? m->part : C, ldc, Cb,
-                                                   bias, relu, ks);
+                                                   bias, relu, ks, mask);
     if (ks > 1) k_gemm_reduce<<<nblk((int64_t) M * ldc), 256, 0, m->s>>>(m->part, ks, (int64_t) M * ldc, C);
     m->flops += 2.0 * M * N * K;
     MCHK(m, hipGetLastError());
@@ -316,13 +361,13 @@ int backward(dacti_mlp *m, int rows, int nb) {
         k_colsum<<<(ldz + 15) / 16, 1024, 0, m->s>>>(m->dZ, ldz, rows, ldz, m->gb[l]);
         MCHK(m, hipGetLastError());
         if (l > 0) {
-            // dA(r, i) = sum_o dZ(r, o) W(o, i); then the ReLU mask of layer l-1
-            rc = gemm(m, m->dZ16, ldz, 1, m->W16[l], m->ip[l], 1, rows, m->ip[l], m->op[l], m->dA, m->ip[l], nullptr,
-                      nullptr, 0);
+            // dZ_{l-1}(r, i) = sum_o dZ(r, o) W(o, i), masked by the ReLU of
+            // layer l-1, into the other dZ buffers
+            rc = gemm(m, m->dZ16, ldz, 1, m->W16[l], m->ip[l], 1, rows, m->ip[l], m->op[l], m->dZb, m->ip[l], m->dZ16b,
+                      nullptr, 0, m->Z[l - 1]);
             if (rc) return rc;
-            const int64_t ne = (int64_t) rows * m->ip[l];
-            k_relu_back<<<nblk(ne), 256, 0, m->s>>>(m->dA, m->Z[l - 1], ne, m->dZ, m->dZ16);
-            MCHK(m, hipGetLastError());
+            std::swap(m->dZ, m->dZb);
+            std::swap(m->dZ16, m->dZ16b);
         }
     }
     return 0;
@@ -382,7 +427,8 @@ void destroy(dacti_mlp *m) {
     for (auto p : m->A16)
         if (p) (void) hipFree(p);
     for (void *p : {(void *) m->X16, (void *) m->dZ, (void *) m->dA, (void *) m->dZ16, (void *) m->yb, (void *) m->part,
-                    (void *) m->lossb})
+                    (void *) m->lossb, (void *) m->dZb, (void *) m->dZ16b, (void *) m->d_adam, (void *) m->d_ctl,
+                    (void *) m->d_c12})
         if (p) (void) hipFree(p);
     if (m->e0) (void) hipEventDestroy(m->e0);
     if (m->e1) (void) hipEventDestroy(m->e1);
@@ -404,8 +450,9 @@ int upload_data(dacti_mlp *m, const float *x, const int32_t *y, int64_t n, float
 }
 
 // batch rows from the device index array idx (nb entries)
-int load_rows(dacti_mlp *m, const float *dx, const int32_t *dy, const int32_t *idx, int nb, int rows) {
-    k_gather<<<(unsigned) rows, 64, 0, m->s>>>(dx, m->n_in, m->n_in, idx, nb, rows, m->X16, m->ip[0], dy, m->yb);
+int load_rows(dacti_mlp *m, const float *dx, const int32_t *dy, const int32_t *idx, int nb, int rows,
+              const TrainCtl *ctl = nullptr) {
+    k_gather<<<(unsigned) rows, 64, 0, m->s>>>(dx, m->n_in, m->n_in, idx, nb, rows, m->X16, m->ip[0], dy, m->yb, ctl);
     MCHK(m, hipGetLastError());
     return 0;
 }
@@ -538,6 +585,24 @@ int dacti_mlp_create(int32_t n_in, int32_t n_hidden, int32_t n_layers, int32_t n
     }
     int rc = put_params(m, p.data());
     if (rc) return fail(rc);
+    // the Adam table: every weight and bias tensor, flat offsets
+    std::vector<AdamT> tab;
+    int64_t off = 0;
+    for (int l = 0; l < m->L; ++l) {
+        const int64_t nw = (int64_t) m->op[l] * m->ip[l];
+        tab.push_back(AdamT{m->W[l], m->gW[l], m->mW[l], m->vW[l], m->W16[l], nw, off});
+        off += nw;
+        tab.push_back(AdamT{m->b[l], m->gb[l], m->mb[l], m->vb[l], nullptr, (int64_t) m->op[l], off});
+        off += m->op[l];
+    }
+    m->n_adam = (int) tab.size();
+    m->adam_total = off;
+    if (hipMalloc((void **) &m->d_adam, sizeof(AdamT) * tab.size()) != hipSuccess ||
+        hipMemcpy(m->d_adam, tab.data(), sizeof(AdamT) * tab.size(), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMalloc((void **) &m->d_ctl, sizeof(TrainCtl)) != hipSuccess) {
+        m->err = "optimizer table allocation failed";
+        return fail(-2);
+    }
     if (hipStreamSynchronize(m->s) != hipSuccess) return fail(-2);
     *out = m;
     return 0;
@@ -647,7 +712,7 @@ int dacti_mlp_forward_noalloc(dacti_mlp *m, const float *x_dev, int64_t n, float
             const int tiles = (rows / 16) * (m->op[l] / 16);
             k_gemm<<<nblk((int64_t) tiles * 64), 256, 0, m->s>>>(a, lda, 1, m->W16[l], 1, m->ip[l], rows, m->op[l],
                                                               m->ip[l], Z, m->op[l], cb, m->b[l],
-                                                              l == L - 1 ? 0 : 1, 1);
+                                                              l == L - 1 ? 0 : 1, 1, nullptr);
             a = cb;
             lda = m->op[l];
         }
@@ -694,6 +759,32 @@ int dacti_mlp_loss_grad(dacti_mlp *m, const float *x, const int32_t *y, int64_t 
     return rc;
 }
 
+namespace {
+
+// One training step (batch of nb rows): gather, forward, backward, loss
+// scatter, Adam over every tensor.  With ctl the batch comes from the
+// device counter (a step replayed from a graph) and the counter advances at
+// the end; otherwise idx already points at the batch and c1, c2 are given.
+int train_step(dacti_mlp *m, const float *dx, const int32_t *dy, const int32_t *idx, int nb, float *lall, float lr,
+               float c1, float c2, const TrainCtl *ctl) {
+    const int rows = pad16(nb);
+    int rc;
+    if ((rc = load_rows(m, dx, dy, idx, nb, rows, ctl)) || (rc = forward(m, rows)) || (rc = backward(m, rows, nb)))
+        return rc;
+    k_scatter_loss<<<nblk(nb), 256, 0, m->s>>>(m->lossb, idx, nb, lall, ctl);
+    k_adam_all<<<nblk(m->adam_total), 256, 0, m->s>>>(m->d_adam, m->n_adam, m->adam_total, lr, c1, c2, ctl,
+                                                      m->d_c12);
+    if (ctl) k_next_batch<<<1, 64, 0, m->s>>>(m->d_ctl);
+    MCHK(m, hipGetLastError());
+    return 0;
+}
+
+}  // namespace
+
+// Mini-batch Adam.  The full batches of an epoch replay one HIP graph of the
+// whole step (captured after the first step, which allocates what the step
+// needs); their batch position and Adam corrections come from device
+// memory.  The first step of a call and a last partial batch run eagerly.
 int dacti_mlp_train(dacti_mlp *m, const float *x, const int32_t *y, int64_t n, int32_t epochs, int32_t batch,
                     float lr, uint64_t seed, float *loss) {
     if (!m || !x || !y || n < 1 || n > INT32_MAX / 2 || epochs < 0 || batch < 16 || (batch % 16) != 0 || batch > 65536) {
@@ -713,38 +804,99 @@ int dacti_mlp_train(dacti_mlp *m, const float *x, const int32_t *y, int64_t n, i
     for (int64_t i = 0; i < n; ++i) perm[(size_t) i] = (int32_t) i;
     int32_t *dperm = nullptr;
     if (!rc) rc = upload_idx(m, perm, &dperm);
+    const int64_t nfull = n / batch;
+    if (!rc && 2 * nfull > m->c12_cap) {
+        if (m->d_c12) (void) hipFree(m->d_c12);
+        m->d_c12 = nullptr;
+        if (hipMalloc((void **) &m->d_c12, sizeof(float) * (size_t) (2 * nfull)) != hipSuccess) {
+            m->err = "correction table allocation failed";
+            rc = -3;
+        } else {
+            m->c12_cap = 2 * nfull;
+        }
+    }
     std::mt19937_64 rng(seed);
+    std::vector<float> c12((size_t) (2 * nfull));
+    hipGraphExec_t gx = nullptr;
+    double step_flops = 0;
     m->flops = 0;
     if (!rc && hipEventRecord(m->e0, m->s) != hipSuccess) rc = -2;
     float last = 0.0f;
+    auto corr = [](int64_t t, float &c1, float &c2) {
+        c1 = 1.0f - std::pow(0.9f, (float) t);
+        c2 = 1.0f - std::pow(0.999f, (float) t);
+    };
     for (int ep = 0; ep < epochs && !rc; ++ep) {
         for (int64_t i = n - 1; i > 0; --i) std::swap(perm[(size_t) i], perm[(size_t) (rng() % (uint64_t) (i + 1))]);
-        // the previous epoch's kernels read dperm: wait, then replace it
+        // the previous epoch's kernels read dperm and the tables: wait, then replace them
+        for (int64_t b = 0; b < nfull; ++b) corr(m->t + b + 1, c12[(size_t) (2 * b)], c12[(size_t) (2 * b + 1)]);
+        const TrainCtl zero{0};
         if (hipStreamSynchronize(m->s) != hipSuccess ||
-            hipMemcpy(dperm, perm.data(), sizeof(int32_t) * (size_t) n, hipMemcpyHostToDevice) != hipSuccess) {
+            hipMemcpy(dperm, perm.data(), sizeof(int32_t) * (size_t) n, hipMemcpyHostToDevice) != hipSuccess ||
+            (nfull > 0 &&
+             hipMemcpy(m->d_c12, c12.data(), sizeof(float) * c12.size(), hipMemcpyHostToDevice) != hipSuccess) ||
+            hipMemcpy(m->d_ctl, &zero, sizeof(TrainCtl), hipMemcpyHostToDevice) != hipSuccess) {
             m->err = "permutation upload failed";
             rc = -2;
             break;
         }
-        for (int64_t s0 = 0; s0 < n && !rc; s0 += batch) {
-            const int nb = (int) std::min<int64_t>(batch, n - s0), rows = pad16(nb);
-            if ((rc = load_rows(m, dx, dy, dperm + s0, nb, rows)) || (rc = forward(m, rows)) ||
-                (rc = backward(m, rows, nb)))
-                break;
-            k_scatter_loss<<<nblk(nb), 256, 0, m->s>>>(m->lossb, dperm + s0, nb, lall);
+        int64_t b0 = 0;
+        if (!gx && nfull > 1) {
+            // first step eagerly, then capture the step
+            float c1, c2;
+            corr(m->t + 1, c1, c2);
+            if ((rc = train_step(m, dx, dy, dperm, batch, lall, lr, c1, c2, nullptr))) break;
             m->t++;
-            const float c1 = 1.0f - std::pow(0.9f, (float) m->t), c2 = 1.0f - std::pow(0.999f, (float) m->t);
-            for (int l = 0; l < m->L; ++l) {
-                const int64_t nw = (int64_t) m->op[l] * m->ip[l];
-                k_adam<<<nblk(nw), 256, 0, m->s>>>(m->W[l], m->gW[l], m->mW[l], m->vW[l], nw, lr, c1, c2);
-                k_adam<<<nblk(m->op[l]), 256, 0, m->s>>>(m->b[l], m->gb[l], m->mb[l], m->vb[l], m->op[l], lr, c1, c2);
-            }
-            if (hipGetLastError() != hipSuccess) {
-                m->err = "optimizer launch failed";
+            const TrainCtl one{1};
+            if (hipMemcpyAsync(m->d_ctl, &one, sizeof(TrainCtl), hipMemcpyHostToDevice, m->s) != hipSuccess ||
+                hipStreamSynchronize(m->s) != hipSuccess) {
+                m->err = "batch counter upload failed";
                 rc = -2;
                 break;
             }
-            rc = refresh_bf16(m);
+            const double f0 = m->flops;
+            hipGraph_t graph = nullptr;
+            if (hipStreamBeginCapture(m->s, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+                m->err = "training step capture failed";
+                rc = -2;
+                break;
+            }
+            rc = train_step(m, dx, dy, dperm, batch, lall, lr, 0.0f, 0.0f, m->d_ctl);
+            const hipError_t e = hipStreamEndCapture(m->s, &graph);
+            if (rc) break;
+            step_flops = m->flops - f0;
+            m->flops = f0;
+            if (e != hipSuccess || hipGraphInstantiate(&gx, graph, nullptr, nullptr, 0) != hipSuccess) {
+                if (graph) (void) hipGraphDestroy(graph);
+                gx = nullptr;
+                m->err = "training step graph failed";
+                rc = -2;
+                break;
+            }
+            (void) hipGraphDestroy(graph);
+            b0 = 1;
+        }
+        for (int64_t b = b0; b < nfull && !rc; ++b) {
+            if (gx) {
+                if (hipGraphLaunch(gx, m->s) != hipSuccess) {
+                    m->err = "training step replay failed";
+                    rc = -2;
+                    break;
+                }
+                m->flops += step_flops;
+            } else {
+                float c1, c2;
+                corr(m->t + 1, c1, c2);
+                rc = train_step(m, dx, dy, dperm + b * batch, batch, lall, lr, c1, c2, nullptr);
+            }
+            m->t++;
+        }
+        if (!rc && nfull * batch < n) {
+            const int nb = (int) (n - nfull * batch);
+            float c1, c2;
+            corr(m->t + 1, c1, c2);
+            rc = train_step(m, dx, dy, dperm + nfull * batch, nb, lall, lr, c1, c2, nullptr);
+            m->t++;
         }
         if (!rc && ep == epochs - 1) {
             std::vector<float> la((size_t) n);
@@ -771,6 +923,7 @@ int dacti_mlp_train(dacti_mlp *m, const float *x, const int32_t *y, int64_t n, i
     }
     if (loss) *loss = last;
     (void) hipStreamSynchronize(m->s);
+    if (gx) (void) hipGraphExecDestroy(gx);
     if (dx) (void) hipFree(dx);
     if (dy) (void) hipFree(dy);
     if (lall) (void) hipFree(lall);

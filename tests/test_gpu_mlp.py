@@ -62,3 +62,33 @@ def test_training_learns_a_linear_teacher():
     print(f"[mlp] loss {l0:.3f} -> {loss:.3f}, held-out accuracy {acc:.3f}, {fl / ms / 1e9:.2f} TFLOP/s", flush=True)
     assert loss < 0.5 * l0 and acc > 0.9
     m.close()
+
+
+def test_adam_steps_match_reference():
+    # every sample identical, so every batch has the same mean gradient and
+    # the trajectory does not depend on the shuffle: 2 epochs of 4 batches
+    # (the first step eager, the others replayed from the step's graph) are
+    # 8 Adam steps on the oracle's gradient.  Adam (torch/Kingma-Ba form, the
+    # one dacti_mlp_train documents): m = 0.9 m + 0.1 g, v = 0.999 v + 0.001
+    # g^2, p -= lr (m / (1 - 0.9^t)) / (sqrt(v / (1 - 0.999^t)) + 1e-8).
+    m = _net(n_in=40, seed=11)
+    g = np.random.default_rng(4)
+    p0 = (m.get_params() + 0.05 * g.standard_normal(m.get_params().shape)).astype(np.float32)
+    m.set_params(p0)
+    x = np.repeat(g.standard_normal((1, 40)).astype(np.float32), 256, axis=0)
+    y = np.full(256, 2, dtype=np.int32)
+    lr = 1e-3
+    m.train(x, y, epochs=2, batch=64, lr=lr, seed=3)
+    p_gpu = m.get_params().astype(np.float64)
+    p = p0.astype(np.float64)
+    mo = np.zeros_like(p)
+    v = np.zeros_like(p)
+    for t in range(1, 9):
+        _, gr = mlp_ref.loss_grad(p.astype(np.float32), m.layer_shapes(), x[:16], y[:16])
+        mo = 0.9 * mo + 0.1 * gr
+        v = 0.999 * v + 0.001 * gr * gr
+        p = p - lr * (mo / (1 - 0.9 ** t)) / (np.sqrt(v / (1 - 0.999 ** t)) + 1e-8)
+    d_gpu, d_ref = p_gpu - p0, p - p0
+    print(f"[mlp] Adam: |update| {np.linalg.norm(d_ref):.3g}, |gpu - ref| {np.linalg.norm(d_gpu - d_ref):.3g}", flush=True)
+    assert np.linalg.norm(d_gpu - d_ref) <= 0.1 * np.linalg.norm(d_ref)
+    m.close()
