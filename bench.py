@@ -520,8 +520,10 @@ def main():
     ap.add_argument("--block", type=int, default=128, help="bubble: threads per list tile with --grid/--graphs")
     ap.add_argument("--dense", type=int, default=1, choices=(0, 1, 2),
                     help="box: flux op by windows for dense clusters (0 never, 1 auto, 2 always)")
-    ap.add_argument("--base", type=int, default=5)
-    ap.add_argument("--extra", type=int, default=4)
+    ap.add_argument("--base", type=int, default=None,
+                    help="drop: base octree level (5); learned3d: 4")
+    ap.add_argument("--extra", type=int, default=None,
+                    help="levels above the base: drop 4, learned3d 3, bubble 4")
     ap.add_argument("--horizon", type=int, default=None, help="learned: 16, learned3d: 8 global steps")
     ap.add_argument("--train-runs", type=int, default=3)
     ap.add_argument("--train-steps", type=int, default=48)
@@ -543,6 +545,13 @@ def main():
         # in global step 6 (at levels 4-8 the oracle breaks down in the same
         # step as the GPU, their states bitwise equal before it): time steps 3-5
         args.steps = {"drop": 3, "learned": 48, "batched": 4}.get(args.workload, 20)
+    # per-workload sizes: the 3D shock-drop at levels 5-9; its learned form at
+    # levels 4-7 (the feature samples of every leaf over 48 steps of three
+    # runs must fit in host memory); the 2D shock-bubble 64x32 + 4 levels
+    if args.base is None:
+        args.base = 4 if args.workload == "learned3d" else 5
+    if args.extra is None:
+        args.extra = 3 if args.workload == "learned3d" else 4
     if args.gpus != 1:
         print(f"bench.py: this engine runs on one GPU; --gpus {args.gpus} is not supported", file=sys.stderr)
         sys.exit(2)
