@@ -529,7 +529,7 @@ def main():
     ap.add_argument("--train-steps", type=int, default=48)
     ap.add_argument("--epochs", type=int, default=10)
     ap.add_argument("--rings", type=int, default=1, help="learned: face-neighbour rings added to predicted refinement")
-    ap.add_argument("--every", type=int, default=16, help="learned3d: remesh cadence of the predictor run")
+    ap.add_argument("--every", type=int, default=8, help="learned3d: remesh cadence of the predictor run")
     ap.add_argument("--cadences", default="", help="learned3d: extra every:rings settings to compare, e.g. 4:1,2:2")
     ap.add_argument("--drop-max-dt", type=float, default=1.6e-7,
                     help="learned3d: global step cap (limits the LTS level spread; wider spreads break down at high "
