@@ -74,3 +74,42 @@ def test_device_step_batch_with_breakdown():
     for o, g in pairs:
         g.close()
         o.close()
+
+
+def test_device_step_gc_and_pool_growth():
+    # everything coarsens in the first step, so the pool is compacted after
+    # it (gc re-bins the levels); with the device-planned substeps the pool
+    # first grows to hold every listed cell's split
+    from tests.helpers import amr_sod_case
+    case = amr_sod_case(base=3, top=8, emin=1e10, emax=1e20)
+    tree = scene.build_tree(case)
+    n0 = tree.n
+    o, g = Oracle(case, tree), Integrator(case, tree)
+    g.set_launch(block=64, grid=64, graphs=True)
+    for s in range(3):
+        print(f"[device step] gc case: step {s}", flush=True)
+        o.integrate()
+        g.integrate()
+        compare_pools(g.get_nodes(), o.export())
+    assert g.n_nodes() < n0 // 4  # compacted
+    g.close()
+    o.close()
+
+
+def test_device_step_refining_sod_grows_pool():
+    # a jump refined from a coarse base: splits every substep, so the pool
+    # grows past its initial capacity while the substeps are planned on the
+    # device
+    from tests.helpers import amr_sod_case
+    case = amr_sod_case(base=4, top=8)
+    tree = scene.build_tree(case)
+    o, g = Oracle(case, tree), Integrator(case, tree)
+    g.set_launch(block=128, grid=32, graphs=False)
+    for s in range(4):
+        print(f"[device step] refining sod: step {s}", flush=True)
+        o.integrate()
+        g.integrate()
+        compare_pools(g.get_nodes(), o.export())
+    assert g.n_nodes() > tree.n
+    g.close()
+    o.close()
