@@ -2394,6 +2394,41 @@ __device__ __forceinline__ uint32_t node_meta(const Dev &d, int64_t i) {
 }
 
 // META: also store every node's metadata word (d.meta) for the window flux op
+// A pool tile (POOL_TILE = 2048 consecutive node ids) is taken by a block
+// of 256 threads, 8 consecutive ids per thread: one 8-byte load per
+// per-node byte array instead of eight 1-byte loads.  Ids at and beyond the
+// pool size count as absent.
+static_assert(POOL_TILE == 256 * 8, "a pool tile is 8 ids per thread of a 256-thread block");
+
+// the 8 bytes of a per-node byte array at ids base8 .. base8 + 7 (base8 a
+// multiple of 8); bytes at ids >= n read as `fill`
+__device__ __forceinline__ uint64_t load8(const GMEM int8_t *a, int64_t base8, int64_t n, uint8_t fill) {
+    if (base8 + 8 <= n) return *(const GMEM uint64_t *) (a + base8);
+    uint64_t v = 0;
+    for (int j = 0; j < 8; ++j) v |= (uint64_t) (base8 + j < n ? (uint8_t) a[base8 + j] : fill) << (8 * j);
+    return v;
+}
+__device__ __forceinline__ uint64_t load8(const GMEM uint8_t *a, int64_t base8, int64_t n, uint8_t fill) {
+    return load8((const GMEM int8_t *) a, base8, n, fill);
+}
+
+// bit j set where byte j of v equals b
+__device__ __forceinline__ unsigned bytes_eq(uint64_t v, uint8_t b) {
+    unsigned m = 0;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) m |= (unsigned) (((v >> (8 * j)) & 0xff) == b) << j;
+    return m;
+}
+
+// the ids of a thread's 8 that belong to the cluster list of level L (id 0,
+// the null node, never does)
+__device__ __forceinline__ unsigned tile_members(const Dev &d, int64_t base8, int64_t n, int L) {
+    unsigned m = bytes_eq(load8(d.cluster, base8, n, 0xff), (uint8_t) L);
+    if (base8 == 0) m &= ~1u;
+    return m;
+}
+
+// META: also store every node's metadata word (d.meta) for the window flux op
 template <int OUT, bool META = false>
 __global__ void __launch_bounds__(256) k_sel_count(const Item *__restrict__ items, const int *__restrict__ toff,
                                                    int n_items, int *tilecnt) {
@@ -2405,14 +2440,28 @@ __global__ void __launch_bounds__(256) k_sel_count(const Item *__restrict__ item
         const Item &it = items[i];
         int cnt = 0;
         if (OUT == 0 ? it.nl > 0 : it.meshing != 0) {
-            const int64_t base = (int64_t) (tile - toff[i]) * POOL_TILE;
-            for (int k = 0; k < POOL_TILE; k += 256) {
-                const int64_t id = base + k + threadIdx.x;
-                const bool pr = id >= 1 && id < it.n && it.d.cluster[id] == it.L;
-                cnt += __popcll(__ballot(pr));
-                if (META && id < it.n) it.d.meta[id] = node_meta(it.d, id);
+            const int64_t base8 = (int64_t) (tile - toff[i]) * POOL_TILE + 8 * (int64_t) threadIdx.x;
+            cnt = __popc(tile_members(it.d, base8, it.n, it.L));
+            if (META && base8 < it.n) {
+                const uint64_t fl = load8(it.d.flags, base8, it.n, 0), lv = load8(it.d.level, base8, it.n, 0),
+                               ac = load8(it.d.acti, base8, it.n, 0), cl = load8(it.d.cluster, base8, it.n, 0);
+                uint32_t w[8];
+#pragma unroll
+                for (int j = 0; j < 8; ++j)
+                    w[j] = (uint32_t) ((fl >> (8 * j)) & 0xff) | (uint32_t) ((lv >> (8 * j)) & 0xff) << 8 |
+                           (uint32_t) ((ac >> (8 * j)) & 0xff) << 16 | (uint32_t) ((cl >> (8 * j)) & 0xff) << 24;
+                if (base8 + 8 <= it.n) {
+                    *(GMEM uint4 *) (it.d.meta + base8) = make_uint4(w[0], w[1], w[2], w[3]);
+                    *(GMEM uint4 *) (it.d.meta + base8 + 4) = make_uint4(w[4], w[5], w[6], w[7]);
+                } else {
+                    for (int j = 0; j < 8; ++j)
+                        if (base8 + j < it.n) it.d.meta[base8 + j] = w[j];
+                }
             }
         }
+        // block sum: wave sums, then the first thread adds the four
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) cnt += __shfl_xor(cnt, o);
         if ((threadIdx.x & 63) == 0) wc[threadIdx.x >> 6] = cnt;
         __syncthreads();
         if (threadIdx.x == 0) tilecnt[tile] = wc[0] + wc[1] + wc[2] + wc[3];
@@ -2456,27 +2505,31 @@ __global__ void __launch_bounds__(256) k_sel_scatter(const Item *__restrict__ it
     __shared__ int wc[4];
     const int total = toff[n_items];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const uint64_t lt = lane ? (~0ull >> (64 - lane)) : 0ull;
     for (int t = blockIdx.x; t < total; t += gridDim.x) {
         const int tile = xcd_block(t, total);
         const int i = find_item(toff, n_items, tile);
         const Item &it = items[i];
         if (OUT == 0 ? it.nl == 0 : it.meshing == 0) continue;  // uniform over the block
         int32_t *out = OUT == 0 ? it.list : it.rest;
-        const int64_t base = (int64_t) (tile - toff[i]) * POOL_TILE;
-        int pos = tileoff[tile];
-        for (int k = 0; k < POOL_TILE; k += 256) {
-            const int64_t id = base + k + threadIdx.x;
-            const bool pr = id >= 1 && id < it.n && it.d.cluster[id] == it.L;
-            const uint64_t m = __ballot(pr);
-            if (lane == 0) wc[w] = __popcll(m);
-            __syncthreads();
-            int before = 0;
-            for (int q = 0; q < w; ++q) before += wc[q];
-            if (pr) out[pos + before + __popcll(m & lt)] = (int32_t) id;
-            pos += wc[0] + wc[1] + wc[2] + wc[3];
-            __syncthreads();
+        const int64_t base8 = (int64_t) (tile - toff[i]) * POOL_TILE + 8 * (int64_t) threadIdx.x;
+        const unsigned m = tile_members(it.d, base8, it.n, it.L);
+        const int c = __popc(m);
+        // inclusive scan of c over the wave, then over the waves: ids keep
+        // their order (thread t holds ids 8t .. 8t + 7 of the tile)
+        int x = c;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const int y = __shfl_up(x, o);
+            if (lane >= o) x += y;
         }
+        if (lane == 63) wc[w] = x;
+        __syncthreads();
+        int pos = tileoff[tile] + x - c;
+        for (int q = 0; q < w; ++q) pos += wc[q];
+#pragma unroll
+        for (int j = 0; j < 8; ++j)
+            if ((m >> j) & 1) out[pos++] = (int32_t) (base8 + j);
+        __syncthreads();
     }
 }
 
@@ -2677,23 +2730,24 @@ static __global__ void __launch_bounds__(256) k_pool_hist(const Item *__restrict
             cur = i;
         }
         const Item &it = items[i];
-        const int64_t base = (int64_t) (tile - toff[i]) * POOL_TILE;
-        for (int k = 0; k < POOL_TILE; k += 256) {
-            const int64_t id = base + k + threadIdx.x;
-            int cl = 16;
-            if (id >= 1 && id < it.n) {
-                cl = it.d.cluster[id];
-                if (cl < 0 || cl >= 16) cl = 16;
-            }
-            // the levels present in the wave, then one count per level
-            unsigned present = cl < 16 ? 1u << cl : 0u;
-            for (int o = 32; o > 0; o >>= 1) present |= (unsigned) __shfl_xor((int) present, o);
-            while (present) {
-                const int l = __ffs(present) - 1;
-                present &= present - 1;
-                const uint64_t m = __ballot(cl == l);
-                if ((threadIdx.x & 63) == 0) atomicAdd(&h[l], (unsigned) __popcll(m));
-            }
+        const int64_t base8 = (int64_t) (tile - toff[i]) * POOL_TILE + 8 * (int64_t) threadIdx.x;
+        // clusters of the thread's 8 ids (absent ids and id 0 as -1)
+        uint64_t v = load8(it.d.cluster, base8, it.n, 0xff);
+        if (base8 == 0) v |= 0xff;
+        unsigned present = 0;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            const unsigned cl = (unsigned) ((v >> (8 * j)) & 0xff);
+            if (cl < 16) present |= 1u << cl;
+        }
+        // the levels present in the wave, then one count per level
+        for (int o = 32; o > 0; o >>= 1) present |= (unsigned) __shfl_xor((int) present, o);
+        while (present) {
+            const int l = __ffs(present) - 1;
+            present &= present - 1;
+            int c = __popc(bytes_eq(v, (uint8_t) l));
+            for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o);
+            if ((threadIdx.x & 63) == 0) atomicAdd(&h[l], (unsigned) c);
         }
     }
     __syncthreads();
