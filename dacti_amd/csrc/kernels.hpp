@@ -2543,8 +2543,12 @@ static __global__ void __launch_bounds__(256) k_pool_flush(const Item *__restric
         const int i = find_item(toff, n_items, tile);
         const Item &it = items[i];
         if (!it.flush) continue;
-        const int64_t base = (int64_t) (tile - toff[i]) * POOL_TILE;
-        for (int k = 0; k < POOL_TILE; k += 256) flush_node(it.d, base + k + threadIdx.x, it.n, it.L);
+        // 8 ids per thread; only ids whose buffer holds level L change
+        const int64_t base8 = (int64_t) (tile - toff[i]) * POOL_TILE + 8 * (int64_t) threadIdx.x;
+        const unsigned m = bytes_eq(load8(it.d.buffer, base8, it.n, 0xff), (uint8_t) it.L);
+        if (m)
+            for (int j = 0; j < 8; ++j)
+                if ((m >> j) & 1) flush_node(it.d, base8 + j, it.n, it.L);
     }
 }
 
