@@ -43,34 +43,49 @@ inline int pad16(int x) { return (x + 15) / 16 * 16; }
 // BF16 copy to Cb.
 // Lane layout of the 16x16x16 BF16 MFMA: lane l holds A(l%16, 4(l/16)+j) and
 // B(4(l/16)+j, l%16) for j < 4, and returns C(4(l/16)+i, l%16) for i < 4.
-// With ks > 1 the reduction is split into ks equal ranges (each a multiple
-// of 16): wave (tile, s) writes its partial tile to C + s * M * ldc, and
-// k_gemm_reduce adds the partials in order (no epilogue in that mode).
+// One wavefront per output tile (k_gemm_ks below splits long reductions).
+// A lane's 4 operand values of one 16-deep slice: one 8-byte load when the
+// reduction index is contiguous (stride 1; every padded dimension is a
+// multiple of 16, so the address is 8-byte aligned), else 4 loads
+template <bool VEC>
+__device__ __forceinline__ s4 load_k4(const uint16_t *p, int64_t sk) {
+    s4 v;
+    if (VEC) {
+        v = *(const s4 *) p;
+    } else {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) v[j] = (short) p[(int64_t) j * sk];
+    }
+    return v;
+}
+
+template <bool AV, bool BV>
 __global__ void __launch_bounds__(256) k_gemm(const uint16_t *__restrict__ A, int64_t sam, int64_t sak,
                                               const uint16_t *__restrict__ B, int64_t sbk, int64_t sbn, int M, int N,
                                               int K, float *__restrict__ C, int64_t ldc, uint16_t *__restrict__ Cb,
-                                              const float *__restrict__ bias, int relu, int ks,
+                                              const float *__restrict__ bias, int relu,
                                               const float *__restrict__ mask) {
     const int wave = (int) ((blockIdx.x * (int64_t) blockDim.x + threadIdx.x) >> 6);
     const int lane = threadIdx.x & 63;
     const int tn = N / 16, tiles = (M / 16) * tn;
-    if (wave >= tiles * ks) return;
-    const int tile = wave % tiles, part = wave / tiles;
-    const int m0 = (tile / tn) * 16, n0 = (tile % tn) * 16;
+    if (wave >= tiles) return;
+    const int m0 = (wave / tn) * 16, n0 = (wave % tn) * 16;
     const int r = lane & 15, kq = (lane >> 4) * 4;
-    const int kl = K / ks, kb = part * kl;
+    const int kl = K;
     f4 c = {0.0f, 0.0f, 0.0f, 0.0f};
-    const uint16_t *pa = A + (int64_t) (m0 + r) * sam + (int64_t) (kb + kq) * sak;
-    const uint16_t *pb = B + (int64_t) (kb + kq) * sbk + (int64_t) (n0 + r) * sbn;
-    C += (int64_t) part * M * ldc;
+    const uint16_t *pa = A + (int64_t) (m0 + r) * sam + (int64_t) kq * sak;
+    const uint16_t *pb = B + (int64_t) kq * sbk + (int64_t) (n0 + r) * sbn;
+    // the next slice's operands are loaded while the current one multiplies
+    s4 a = load_k4<AV>(pa, sak), b = load_k4<BV>(pb, sbk);
     for (int k0 = 0; k0 < kl; k0 += 16) {
-        s4 a, b;
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            a[j] = (short) pa[(int64_t) (k0 + j) * sak];
-            b[j] = (short) pb[(int64_t) (k0 + j) * sbk];
+        s4 an = a, bn = b;
+        if (k0 + 16 < kl) {
+            an = load_k4<AV>(pa + (int64_t) (k0 + 16) * sak, sak);
+            bn = load_k4<BV>(pb + (int64_t) (k0 + 16) * sbk, sbk);
         }
         c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
+        a = an;
+        b = bn;
     }
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
@@ -84,13 +99,60 @@ __global__ void __launch_bounds__(256) k_gemm(const uint16_t *__restrict__ A, in
     }
 }
 
-// C[i] = sum over s of part[s * n + i], in s order
-__global__ void k_gemm_reduce(const float *__restrict__ part, int ks, int64_t n, float *__restrict__ C) {
-    const int64_t i = blockIdx.x * (int64_t) blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    float v = part[i];
-    for (int s = 1; s < ks; ++s) v += part[(int64_t) s * n + i];
-    C[i] = v;
+// The weight gradient's long reduction (K = batch rows) split over the ks
+// waves of one block (blockDim = 64 ks, one 16x16 output tile per block):
+// wave s reduces the s-th range of K, and wave 0 adds the partial tiles in
+// s order, the order of a serial sum of the ranges.  No epilogue.
+template <bool AV, bool BV>
+__global__ void __launch_bounds__(1024) k_gemm_ks(const uint16_t *__restrict__ A, int64_t sam, int64_t sak,
+                                                  const uint16_t *__restrict__ B, int64_t sbk, int64_t sbn, int M,
+                                                  int N, int K, float *__restrict__ C, int64_t ldc) {
+    __shared__ f4 part[16][64];
+    const int ks = (int) (blockDim.x >> 6), s = (int) (threadIdx.x >> 6), lane = threadIdx.x & 63;
+    const int tn = N / 16, tile = (int) blockIdx.x;
+    if (tile >= (M / 16) * tn) return;
+    const int m0 = (tile / tn) * 16, n0 = (tile % tn) * 16;
+    const int r = lane & 15, kq = (lane >> 4) * 4;
+    const int kl = K / ks, kb = s * kl;
+    f4 c = {0.0f, 0.0f, 0.0f, 0.0f};
+    const uint16_t *pa = A + (int64_t) (m0 + r) * sam + (int64_t) (kb + kq) * sak;
+    const uint16_t *pb = B + (int64_t) (kb + kq) * sbk + (int64_t) (n0 + r) * sbn;
+    s4 a = load_k4<AV>(pa, sak), b = load_k4<BV>(pb, sbk);
+    for (int k0 = 0; k0 < kl; k0 += 16) {
+        s4 an = a, bn = b;
+        if (k0 + 16 < kl) {
+            an = load_k4<AV>(pa + (int64_t) (k0 + 16) * sak, sak);
+            bn = load_k4<BV>(pb + (int64_t) (k0 + 16) * sbk, sbk);
+        }
+        c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
+        a = an;
+        b = bn;
+    }
+    part[s][lane] = c;
+    __syncthreads();
+    if (s != 0) return;
+    f4 v = part[0][lane];
+    for (int q = 1; q < ks; ++q) {
+        const f4 w = part[q][lane];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) v[i] += w[i];
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i) C[(int64_t) (m0 + kq + i) * ldc + n0 + r] = v[i];
+}
+
+// launch k_gemm with the load form of each operand
+inline void launch_gemm(unsigned blocks, hipStream_t s, const uint16_t *A, int64_t sam, int64_t sak,
+                        const uint16_t *B, int64_t sbk, int64_t sbn, int M, int N, int K, float *C, int64_t ldc,
+                        uint16_t *Cb, const float *bias, int relu, const float *mask) {
+    if (sak == 1 && sbk == 1)
+        k_gemm<true, true><<<blocks, 256, 0, s>>>(A, sam, sak, B, sbk, sbn, M, N, K, C, ldc, Cb, bias, relu, mask);
+    else if (sak == 1)
+        k_gemm<true, false><<<blocks, 256, 0, s>>>(A, sam, sak, B, sbk, sbn, M, N, K, C, ldc, Cb, bias, relu, mask);
+    else if (sbk == 1)
+        k_gemm<false, true><<<blocks, 256, 0, s>>>(A, sam, sak, B, sbk, sbn, M, N, K, C, ldc, Cb, bias, relu, mask);
+    else
+        k_gemm<false, false><<<blocks, 256, 0, s>>>(A, sam, sak, B, sbk, sbn, M, N, K, C, ldc, Cb, bias, relu, mask);
 }
 
 __global__ void k_to_bf16(const float *__restrict__ x, uint16_t *__restrict__ y, int64_t n) {
@@ -149,25 +211,36 @@ __global__ void k_softmax_ce(const float *__restrict__ z, int ldz, int nout, con
     }
 }
 
-// db[o] = sum over rows of dz[r][o] (fixed order)
-// out[o] = sum over the rows of dz[r*ld + o].  A block takes 16 columns
+// out[o] = sum over the rows of dz[r*ld + o], for every layer's bias in one
+// launch.  A block takes 16 columns
 // with 64 lanes each: lane y adds rows y, y+64, ... (coalesced across the
 // columns) and the first lane of a column adds the 64 partial sums in lane
 // order, so the result does not depend on timing.
-__global__ void __launch_bounds__(1024) k_colsum(const float *__restrict__ dz, int ld, int rows, int ncol,
-                                                 float *__restrict__ out) {
+// one layer's bias gradient in the all-layers column sum: its dZ, leading
+// dimension (= columns) and output, and its first block
+struct ColsumT {
+    const float *dz;
+    int ld;
+    float *out;
+    int blk0;
+};
+
+__global__ void __launch_bounds__(1024) k_colsum(const ColsumT *__restrict__ tab, int nl, int rows) {
     __shared__ float part[64][17];
+    int l = 0;
+    while (l + 1 < nl && tab[l + 1].blk0 <= (int) blockIdx.x) ++l;
+    const ColsumT &t = tab[l];
     const int x = threadIdx.x & 15, y = threadIdx.x >> 4;
-    const int o = blockIdx.x * 16 + x;
+    const int o = ((int) blockIdx.x - t.blk0) * 16 + x;
     float s = 0.0f;
-    if (o < ncol)
-        for (int r = y; r < rows; r += 64) s += dz[(int64_t) r * ld + o];
+    if (o < t.ld)
+        for (int r = y; r < rows; r += 64) s += t.dz[(int64_t) r * t.ld + o];
     part[y][x] = s;
     __syncthreads();
-    if (y == 0 && o < ncol) {
-        float t = part[0][x];
-        for (int k = 1; k < 64; ++k) t += part[k][x];
-        out[o] = t;
+    if (y == 0 && o < t.ld) {
+        float v = part[0][x];
+        for (int k = 1; k < 64; ++k) v += part[k][x];
+        t.out[o] = v;
     }
 }
 
@@ -236,14 +309,14 @@ struct dacti_mlp {
     uint16_t *X16 = nullptr;
     std::vector<float *> Z;
     std::vector<uint16_t *> A16;
-    float *dZ = nullptr, *dA = nullptr, *lossb = nullptr;
-    float *part = nullptr;  // split-reduction partial tiles
-    int64_t part_cap = 0;
-    uint16_t *dZ16 = nullptr;
-    // the next layer's dZ while the current one is read (the masked input
-    // gradient is written by the product that reads dZ)
-    float *dZb = nullptr;
-    uint16_t *dZ16b = nullptr;
+    float *lossb = nullptr;
+    // each layer's dZ (FP32 and BF16): the masked input gradient of layer
+    // l is written into layer l-1's while layer l's is read, and the bias
+    // gradients of all layers are column sums of them, taken at the end
+    std::vector<float *> dZl;
+    std::vector<uint16_t *> dZ16l;
+    ColsumT *d_cs = nullptr;  // per-layer column-sum table (device)
+    int cs_blocks = 0;
     int32_t *yb = nullptr;
     AdamT *d_adam = nullptr;  // every parameter tensor, for k_adam_all
     int n_adam = 0;
@@ -285,24 +358,35 @@ int ensure_rows(dacti_mlp *m, int rows) {
     auto fr = [](void *p) {
         if (p) (void) hipFree(p);
     };
-    fr(m->X16), fr(m->dZ), fr(m->dA), fr(m->dZ16), fr(m->yb), fr(m->lossb), fr(m->dZb), fr(m->dZ16b);
+    fr(m->X16), fr(m->yb), fr(m->lossb), fr(m->d_cs);
+    m->d_cs = nullptr;
     for (auto p : m->Z) fr(p);
     for (auto p : m->A16) fr(p);
+    for (auto p : m->dZl) fr(p);
+    for (auto p : m->dZ16l) fr(p);
     const int cap = pad16(rows);
     int maxw = 16;
     for (int l = 0; l < m->L; ++l) maxw = std::max({maxw, m->ip[l], m->op[l]});
     int rc;
-    if ((rc = dalloc(m, &m->X16, (size_t) cap * m->ip[0])) || (rc = dalloc(m, &m->dZ, (size_t) cap * maxw)) ||
-        (rc = dalloc(m, &m->dA, (size_t) cap * maxw)) || (rc = dalloc(m, &m->dZ16, (size_t) cap * maxw)) ||
-        (rc = dalloc(m, &m->dZb, (size_t) cap * maxw)) || (rc = dalloc(m, &m->dZ16b, (size_t) cap * maxw)) ||
-        (rc = dalloc(m, &m->yb, (size_t) cap)) ||
+    if ((rc = dalloc(m, &m->X16, (size_t) cap * m->ip[0])) || (rc = dalloc(m, &m->yb, (size_t) cap)) ||
         (rc = dalloc(m, &m->lossb, (size_t) cap)))
         return rc;
     m->Z.assign(m->L, nullptr);
     m->A16.assign(m->L, nullptr);
-    for (int l = 0; l < m->L; ++l)
-        if ((rc = dalloc(m, &m->Z[l], (size_t) cap * m->op[l])) || (rc = dalloc(m, &m->A16[l], (size_t) cap * m->op[l])))
+    m->dZl.assign(m->L, nullptr);
+    m->dZ16l.assign(m->L, nullptr);
+    std::vector<ColsumT> cs;
+    int blk = 0;
+    for (int l = 0; l < m->L; ++l) {
+        if ((rc = dalloc(m, &m->Z[l], (size_t) cap * m->op[l])) || (rc = dalloc(m, &m->A16[l], (size_t) cap * m->op[l])) ||
+            (rc = dalloc(m, &m->dZl[l], (size_t) cap * m->op[l])) || (rc = dalloc(m, &m->dZ16l[l], (size_t) cap * m->op[l])))
             return rc;
+        cs.push_back(ColsumT{m->dZl[l], m->op[l], m->gb[l], blk});
+        blk += (m->op[l] + 15) / 16;
+    }
+    m->cs_blocks = blk;
+    MCHK(m, hipMalloc((void **) &m->d_cs, sizeof(ColsumT) * cs.size()));
+    MCHK(m, hipMemcpy(m->d_cs, cs.data(), sizeof(ColsumT) * cs.size(), hipMemcpyHostToDevice));
     m->cap = cap;
     return 0;
 }
@@ -314,18 +398,20 @@ int gemm(dacti_mlp *m, const uint16_t *A, int64_t sam, int64_t sak, const uint16
     const int64_t tiles = (int64_t) (M / 16) * (N / 16);
     int ks = 1;
     if (!bias && !relu && !Cb && !mask)
-        while (ks < 32 && tiles * ks < 2048 && K / (2 * ks) >= 64 && (K / 16) % (2 * ks) == 0) ks *= 2;
-    const int64_t need = (int64_t) ks * M * ldc;
-    if (ks > 1 && need > m->part_cap) {
-        MCHK(m, hipStreamSynchronize(m->s));
-        if (m->part) (void) hipFree(m->part);
-        m->part = nullptr;
-        MCHK(m, hipMalloc((void **) &m->part, sizeof(float) * (size_t) need));
-        m->part_cap = need;
+        while (ks < 16 && tiles * ks < 2048 && K / (2 * ks) >= 64 && (K / 16) % (2 * ks) == 0) ks *= 2;
+    if (ks > 1) {
+        const unsigned nb = (unsigned) tiles, nt = 64u * (unsigned) ks;
+        if (sak == 1 && sbk == 1)
+            k_gemm_ks<true, true><<<nb, nt, 0, m->s>>>(A, sam, sak, B, sbk, sbn, M, N, K, C, ldc);
+        else if (sak == 1)
+            k_gemm_ks<true, false><<<nb, nt, 0, m->s>>>(A, sam, sak, B, sbk, sbn, M, N, K, C, ldc);
+        else if (sbk == 1)
+            k_gemm_ks<false, true><<<nb, nt, 0, m->s>>>(A, sam, sak, B, sbk, sbn, M, N, K, C, ldc);
+        else
+            k_gemm_ks<false, false><<<nb, nt, 0, m->s>>>(A, sam, sak, B, sbk, sbn, M, N, K, C, ldc);
+    } else {
+        launch_gemm(nblk(tiles * 64), m->s, A, sam, sak, B, sbk, sbn, M, N, K, C, ldc, Cb, bias, relu, mask);
     }
-    k_gemm<<<nblk(tiles * ks * 64), 256, 0, m->s>>>(A, sam, sak, B, sbk, sbn, M, N, K, ks > 1 ? m->part : C, ldc, Cb,
-                                                   bias, relu, ks, mask);
-    if (ks > 1) k_gemm_reduce<<<nblk((int64_t) M * ldc), 256, 0, m->s>>>(m->part, ks, (int64_t) M * ldc, C);
     m->flops += 2.0 * M * N * K;
     MCHK(m, hipGetLastError());
     return 0;
@@ -350,26 +436,25 @@ int forward(dacti_mlp *m, int rows) {
 int backward(dacti_mlp *m, int rows, int nb) {
     const int L = m->L;
     k_softmax_ce<<<nblk(rows), 256, 0, m->s>>>(m->Z[L - 1], m->op[L - 1], m->n_out, m->yb, rows, 1.0f / (float) nb,
-                                              m->dZ, m->dZ16, m->lossb);
+                                              m->dZl[L - 1], m->dZ16l[L - 1], m->lossb);
     MCHK(m, hipGetLastError());
     for (int l = L - 1; l >= 0; --l) {
         const uint16_t *aprev = l > 0 ? m->A16[l - 1] : m->X16;
         const int ldp = m->ip[l], ldz = m->op[l];
         // gW_l(o, i) = sum_r dZ(r, o) A_prev(r, i)
-        int rc = gemm(m, m->dZ16, 1, ldz, aprev, ldp, 1, m->op[l], m->ip[l], rows, m->gW[l], m->ip[l], nullptr, nullptr, 0);
+        int rc = gemm(m, m->dZ16l[l], 1, ldz, aprev, ldp, 1, m->op[l], m->ip[l], rows, m->gW[l], m->ip[l], nullptr,
+                      nullptr, 0);
         if (rc) return rc;
-        k_colsum<<<(ldz + 15) / 16, 1024, 0, m->s>>>(m->dZ, ldz, rows, ldz, m->gb[l]);
-        MCHK(m, hipGetLastError());
         if (l > 0) {
-            // dZ_{l-1}(r, i) = sum_o dZ(r, o) W(o, i), masked by the ReLU of
-            // layer l-1, into the other dZ buffers
-            rc = gemm(m, m->dZ16, ldz, 1, m->W16[l], m->ip[l], 1, rows, m->ip[l], m->op[l], m->dZb, m->ip[l], m->dZ16b,
-                      nullptr, 0, m->Z[l - 1]);
+            // dZ_{l-1}(r, i) = sum_o dZ_l(r, o) W(o, i), masked by the ReLU of layer l-1
+            rc = gemm(m, m->dZ16l[l], ldz, 1, m->W16[l], m->ip[l], 1, rows, m->ip[l], m->op[l], m->dZl[l - 1], m->ip[l],
+                      m->dZ16l[l - 1], nullptr, 0, m->Z[l - 1]);
             if (rc) return rc;
-            std::swap(m->dZ, m->dZb);
-            std::swap(m->dZ16, m->dZ16b);
         }
     }
+    // gb_l(o) = sum_r dZ_l(r, o), every layer
+    k_colsum<<<(unsigned) m->cs_blocks, 1024, 0, m->s>>>(m->d_cs, L, rows);
+    MCHK(m, hipGetLastError());
     return 0;
 }
 
@@ -426,9 +511,12 @@ void destroy(dacti_mlp *m) {
         if (p) (void) hipFree(p);
     for (auto p : m->A16)
         if (p) (void) hipFree(p);
-    for (void *p : {(void *) m->X16, (void *) m->dZ, (void *) m->dA, (void *) m->dZ16, (void *) m->yb, (void *) m->part,
-                    (void *) m->lossb, (void *) m->dZb, (void *) m->dZ16b, (void *) m->d_adam, (void *) m->d_ctl,
-                    (void *) m->d_c12})
+    for (auto p : m->dZl)
+        if (p) (void) hipFree(p);
+    for (auto p : m->dZ16l)
+        if (p) (void) hipFree(p);
+    for (void *p : {(void *) m->X16, (void *) m->yb, (void *) m->lossb, (void *) m->d_adam, (void *) m->d_ctl,
+                    (void *) m->d_c12, (void *) m->d_cs})
         if (p) (void) hipFree(p);
     if (m->e0) (void) hipEventDestroy(m->e0);
     if (m->e1) (void) hipEventDestroy(m->e1);
@@ -710,9 +798,9 @@ int dacti_mlp_forward_noalloc(dacti_mlp *m, const float *x_dev, int64_t n, float
         for (int l = 0; l < L; ++l) {
             uint16_t *cb = (l % 2 == 0) ? Aping : Apong;
             const int tiles = (rows / 16) * (m->op[l] / 16);
-            k_gemm<<<nblk((int64_t) tiles * 64), 256, 0, m->s>>>(a, lda, 1, m->W16[l], 1, m->ip[l], rows, m->op[l],
+            launch_gemm(nblk((int64_t) tiles * 64), m->s, a, lda, 1, m->W16[l], 1, m->ip[l], rows, m->op[l],
                                                               m->ip[l], Z, m->op[l], cb, m->b[l],
-                                                              l == L - 1 ? 0 : 1, 1, nullptr);
+                                                              l == L - 1 ? 0 : 1, nullptr);
             a = cb;
             lda = m->op[l];
         }
