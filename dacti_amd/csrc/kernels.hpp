@@ -675,16 +675,24 @@ __device__ void deposit_coarse(const Dev &d, int32_t y, int L) {
 // same-tree-level leaves (every cell of a uniform tree, most cells of an
 // adaptive one): one flux evaluation site per face, no sub-face walks.
 // A cell with any other face is appended to `rest` (order irrelevant:
-// cells are independent) and handled by OpFluxRest.
+// cells are independent) and handled by OpFluxRest.  A face between two
+// listed cells of one wavefront (consecutive ids sit in consecutive lanes:
+// siblings, runs of a level) is evaluated once, by its owner as its + face,
+// and handed to the other cell by a lane permute; each cell still adds its
+// faces in face order, so the sums are those of evaluating it twice.
 template <int MODEL, int D>
 __device__ __forceinline__ void flux_simple_body(const Dev &d, const MConst &c, const int32_t *__restrict__ list, int nl, int li,
                                                       int L, double dt, int32_t *rest, int32_t *n_rest) {
     using S = Sz<MODEL, D>;
     constexpr int NF = S::NF, NE = S::NE;
-    if (li >= nl) return;
+    // every lane of the wavefront stays to the end (the face fluxes handed
+    // between lanes need all of them); act: the lane has a listed cell of
+    // this op
+    const int lane = (int) (threadIdx.x & 63);
     const int64_t C = d.C;
-    const int32_t y = list[li];
-    const int ylev = d.level[y];
+    const bool valid = li < nl;
+    const int32_t y = valid ? list[li] : -1;
+    const int ylev = valid ? d.level[y] : 0;
     // classify: every face a boundary or a same-level leaf?  The neighbour
     // ids and their LTS level and cluster are gathered here for all faces at
     // once, so the face loop below waits only for the neighbour states.
@@ -693,103 +701,165 @@ __device__ __forceinline__ void flux_simple_body(const Dev &d, const MConst &c, 
     int nac[2 * D], ncls[2 * D];
 #pragma unroll
     for (int f = 0; f < 2 * D; ++f) {
-        const int32_t nb = d.nbr[f * C + y];
+        const int32_t nb = valid ? d.nbr[f * C + y] : 0;
         nbs[f] = nb;
         nac[f] = nb ? d.acti[nb] : 0;
         ncls[f] = nb ? d.cluster[nb] : -1;
         if (nb != 0 && (!(d.flags[nb] & F_LEAF) || d.level[nb] != ylev)) simple = false;
     }
-    if (!simple) {
-        rest[atomicAdd(n_rest, 1)] = y;
-        return;
-    }
-    const double ys = d.size[y];
+    if (valid && !simple) rest[atomicAdd(n_rest, 1)] = y;
+    const bool act = valid && simple;
+    if (!__any(act)) return;  // uniform over the wavefront
+    const double ys = act ? d.size[y] : 0.0;
     const double area = c.area[ylev];
     Side<MODEL, D> sy;
+    if (act) {
 #pragma unroll
-    for (int e = 0; e < NE; ++e) {
-        sy.p[e] = d.p[e * C + y];
-        sy.r[e] = d.rhs[e * C + y];
-    }
-    sy.t = d.t[y];
-    if constexpr (MODEL == MULTI) sy.kap = d.k[S::IKAPPA * C + y];
-    if constexpr (S::NG > 0) {
-        sy.g = d.grad + y;
-        sy.gC = C;
+        for (int e = 0; e < NE; ++e) {
+            sy.p[e] = d.p[e * C + y];
+            sy.r[e] = d.rhs[e * C + y];
+        }
+        sy.t = d.t[y];
+        if constexpr (MODEL == MULTI) sy.kap = d.k[S::IKAPPA * C + y];
+        if constexpr (S::NG > 0) {
+            sy.g = d.grad + y;
+            sy.gC = C;
+        }
     }
     double tot[NF];
 #pragma unroll
     for (int e = 0; e < NF; ++e) tot[e] = 0.0;
 #pragma unroll 1
     for (int dim = 0; dim < D; ++dim) {
+        double yc = 0.0;
+        if (act) {
 #pragma unroll
-        for (int e = 0; e < NE; ++e) sy.s[e] = d.slope[(dim * NE + e) * C + y];
-        if constexpr (MODEL == MULTI) sy.kn = d.k[(S::IKN + dim) * C + y];
-        const double yc = d.center[dim * C + y];
-#pragma unroll 1
-        for (int dir = 0; dir < 2; ++dir) {
-            const int f = 2 * dim + dir;
-            const int32_t nb = pick_i<2 * D>(nbs, f);
-            double cc[NF], F[NF];
+            for (int e = 0; e < NE; ++e) sy.s[e] = d.slope[(dim * NE + e) * C + y];
+            if constexpr (MODEL == MULTI) sy.kn = d.k[(S::IKN + dim) * C + y];
+            yc = d.center[dim * C + y];
+        }
+        // the face flux of (dim, dir) from the owner's side: y's own face
+        // (y_computes) or the neighbour's face toward y (impl/flux.hpp:58-91)
+        auto eval = [&](int dir, int32_t nb, bool y_computes, double *F) {
+            Side<MODEL, D> sn;
+            load_side<MODEL, D>(d, nb, dim, sn);
+            const double nbc = d.center[dim * C + nb];
+            // the owner's face centre, time interval and periodic image
+            const double oc = y_computes ? yc : nbc;
+            const int odir = y_computes ? dir : 1;
+            const double fc = oc + ys * (odir - 0.5);
+            const double t0 = y_computes ? sy.t : sn.t;
+            const double t1 = t0 + dt;
+            double other = y_computes ? nbc : yc;
+            if (c.periodic[dim]) {
+                if (odir == 1) {
+                    if (oc + ys > 0.5 * c.active_size[dim]) other += c.active_size[dim];
+                } else {
+                    if (oc - ys < -0.5 * c.active_size[dim]) other -= c.active_size[dim];
+                }
+            }
+            const double cy = y_computes ? yc : other;
+            const double cn = y_computes ? other : nbc;
+            if (dir == 1) model_flux_axis<MODEL, D>(c, sy, cy, sn, cn, fc, area, t0, t1, dim, F);
+            else model_flux_axis<MODEL, D>(c, sn, cn, sy, cy, fc, area, t0, t1, dim, F);
+        };
+        // + face first: y evaluates it unless the neighbour is on a finer
+        // LTS level (then its deposit is received)
+        const int fm = 2 * dim, fp = 2 * dim + 1;
+        double ccp[NF], F[NF];
 #pragma unroll
-            for (int e = 0; e < NF; ++e) cc[e] = 0.0;
-            if (nb == 0) {
-                boundary_flux<MODEL, D>(d, c, y, f, dim, dir, dt, F);
+        for (int e = 0; e < NF; ++e) ccp[e] = 0.0, F[e] = 0.0;
+        const int32_t nbp = pick_i<2 * D>(nbs, fp);
+        bool shares = false;  // F holds y's + face flux toward a neighbour of the cluster
+        if (!act) {
+        } else if (nbp == 0) {
+            boundary_flux<MODEL, D>(d, c, y, fp, dim, 1, dt, F);
 #pragma unroll
-                for (int e = 0; e < NF; ++e) cc[e] = cc[e] - F[e];
+            for (int e = 0; e < NF; ++e) ccp[e] = ccp[e] - F[e];
+        } else {
+            const int nacti = pick_i<2 * D>(nac, fp);
+            const int ncl = pick_i<2 * D>(ncls, fp);
+            if (nacti > L) {
+#pragma unroll
+                for (int e = 0; e < NF; ++e) {
+                    ccp[e] = d.acc[(fp * NF + e) * C + y];
+                    d.acc[(fp * NF + e) * C + y] = 0.0;
+                }
             } else {
-                const int nacti = pick_i<2 * D>(nac, f);
-                const int ncl = pick_i<2 * D>(ncls, f);
-                if (nacti > L) {
+                eval(1, nbp, true, F);
+#pragma unroll
+                for (int e = 0; e < NF; ++e) ccp[e] = ccp[e] - F[e];
+                if (ncl != L) {
 #pragma unroll
                     for (int e = 0; e < NF; ++e) {
-                        cc[e] = d.acc[(f * NF + e) * C + y];
-                        d.acc[(f * NF + e) * C + y] = 0.0;
+                        GMEM double &a = d.acc[(fm * NF + e) * C + nbp];
+                        a = a + F[e];
                     }
+                } else {
+                    shares = true;
                 }
-                const bool y_computes = nacti <= L && (dir == 1 || nacti < L);
-                const bool nb_computes = !y_computes && ncl == L && dir == 0 && d.nbr[(2 * dim + 1) * C + nb] == y;
-                if (y_computes || nb_computes) {
-                    Side<MODEL, D> sn;
-                    load_side<MODEL, D>(d, nb, dim, sn);
-                    const double nbc = d.center[dim * C + nb];
-                    // the owner's face centre, time interval and periodic image
-                    const double oc = y_computes ? yc : nbc;
-                    const int odir = y_computes ? dir : 1;
-                    const double fc = oc + ys * (odir - 0.5);
-                    const double t0 = y_computes ? sy.t : sn.t;
-                    const double t1 = t0 + dt;
-                    double other = y_computes ? nbc : yc;
-                    if (c.periodic[dim]) {
-                        if (odir == 1) {
-                            if (oc + ys > 0.5 * c.active_size[dim]) other += c.active_size[dim];
-                        } else {
-                            if (oc - ys < -0.5 * c.active_size[dim]) other -= c.active_size[dim];
-                        }
-                    }
-                    const double cy = y_computes ? yc : other;
-                    const double cn = y_computes ? other : nbc;
-                    if (dir == 1) model_flux_axis<MODEL, D>(c, sy, cy, sn, cn, fc, area, t0, t1, dim, F);
-                    else model_flux_axis<MODEL, D>(c, sn, cn, sy, cy, fc, area, t0, t1, dim, F);
-                    // y receives -F on its + face and +F on its - face
+            }
+        }
+        // the - neighbour's + face flux, when the - neighbour is listed in
+        // this wavefront (consecutive ids are consecutive list entries) and
+        // evaluated that face toward y (every lane takes part)
+        const int32_t nbm = pick_i<2 * D>(nbs, fm);
+        const int64_t dl = (int64_t) y - nbm;
+        const int src = nbm != 0 && dl > 0 && dl <= lane ? lane - (int) dl : lane;
+        const int32_t src_id = __shfl(y, src);
+        const int32_t src_nbp = __shfl(nbp, src);
+        const int src_shares = __shfl((int) shares, src);
+        double X[NF];
 #pragma unroll
-                    for (int e = 0; e < NF; ++e) cc[e] = dir == 1 ? cc[e] - F[e] : cc[e] + F[e];
-                    if (y_computes && ncl != L) {
-                        const int fo = 2 * dim + (1 - dir);
+        for (int e = 0; e < NF; ++e) X[e] = __shfl(F[e], src);
+        const bool handed = src != lane && src_id == nbm && src_nbp == y && src_shares;
+        double ccm[NF];
 #pragma unroll
-                        for (int e = 0; e < NF; ++e) {
-                            GMEM double &a = d.acc[(fo * NF + e) * C + nb];
-                            a = dir == 1 ? a + F[e] : a - F[e];
-                        }
+        for (int e = 0; e < NF; ++e) ccm[e] = 0.0;
+        if (!act) {
+        } else if (nbm == 0) {
+            boundary_flux<MODEL, D>(d, c, y, fm, dim, 0, dt, F);
+#pragma unroll
+            for (int e = 0; e < NF; ++e) ccm[e] = ccm[e] - F[e];
+        } else {
+            const int nacti = pick_i<2 * D>(nac, fm);
+            const int ncl = pick_i<2 * D>(ncls, fm);
+            if (nacti > L) {
+#pragma unroll
+                for (int e = 0; e < NF; ++e) {
+                    ccm[e] = d.acc[(fm * NF + e) * C + y];
+                    d.acc[(fm * NF + e) * C + y] = 0.0;
+                }
+            }
+            const bool y_computes = nacti < L;
+            const bool nb_computes = !y_computes && ncl == L && (handed || d.nbr[fp * C + nbm] == y);
+            if (nb_computes && handed) {
+                // the same evaluation (owner nbm, its + face) made by nbm's lane
+#pragma unroll
+                for (int e = 0; e < NF; ++e) ccm[e] = ccm[e] + X[e];
+            } else if (y_computes || nb_computes) {
+                eval(0, nbm, y_computes, F);
+#pragma unroll
+                for (int e = 0; e < NF; ++e) ccm[e] = ccm[e] + F[e];
+                if (y_computes && ncl != L) {
+#pragma unroll
+                    for (int e = 0; e < NF; ++e) {
+                        GMEM double &a = d.acc[(fp * NF + e) * C + nbm];
+                        a = a - F[e];
                     }
                 }
             }
-#pragma unroll
-            for (int e = 0; e < NF; ++e) tot[e] = tot[e] + cc[e];
         }
-    }
+        // y's faces in their order: - then +
 #pragma unroll
-    for (int e = 0; e < NF; ++e) d.tot[e * C + y] = tot[e];
+        for (int e = 0; e < NF; ++e) tot[e] = tot[e] + ccm[e];
+#pragma unroll
+        for (int e = 0; e < NF; ++e) tot[e] = tot[e] + ccp[e];
+    }
+    if (act) {
+#pragma unroll
+        for (int e = 0; e < NF; ++e) d.tot[e * C + y] = tot[e];
+    }
 }
 
 // ---------------------------------------------------------------- update
