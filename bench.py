@@ -105,6 +105,34 @@ def cpu_baseline(case, tree, seconds_target=20.0):
                       f"({n} leaf-cell updates in {el:.1f} s, OpenMP {threads} threads)"}
 
 
+def oracle_steps_baseline(case, tree, max_steps, seconds=10.0):
+    """The oracle (OpenMP) stepping one case from its initial tree, global
+    step after global step (remeshing in every substep), until about
+    `seconds` of CPU work or `max_steps` steps."""
+    from oracle.binding import Oracle
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    o = Oracle(case, tree)
+    t1 = time.perf_counter()
+    n = 0
+    while time.perf_counter() - t1 < seconds and n < max_steps:
+        o.integrate()
+        n += 1
+    el = time.perf_counter() - t1
+    out = {"value": o.cell_updates / el, "unit": "cell-updates/s", "cores": threads, "kind": "port",
+           "sample": f"the first {n} global steps of the held-out case from the same initial state, remeshing in "
+                     f"every substep ({o.cell_updates} updates in {el:.1f} s, OpenMP {threads} threads)"}
+    o.close()
+    return out
+
+
+def mlp_roofline(gemm_tflops):
+    """MFMA roofline of the predictor's training: FLOPs of its GEMMs over the
+    training time, against the dense BF16 peak."""
+    return {"bound": "mfma", "achieved": gemm_tflops, "peak": 2500.0, "unit": "TFLOP/s",
+            "frac": gemm_tflops / 2500.0, "traffic": None,
+            "kernel": "MLP training (BF16 MFMA products, FP32 accumulation, whole training time)"}
+
+
 def copy_probe(gib=1.0):
     """A plain device-to-device copy of known size (HIP runtime), run in the
     same process so hardware byte counters can be calibrated against it."""
@@ -282,8 +310,8 @@ def learned(args):
         # the same simulated time in every run: wall-time ratios are time-to-solution speedups
         "learned_speedup_vs_every_4": t_p4 / t_ln,
         "learned_speedup_vs_every_substep": t_ref / t_ln,
-        "roofline": None,
-        "cpu_baseline": None,
+        "roofline": mlp_roofline(fl / ms / 1e9),
+        "cpu_baseline": None if args.no_cpu_baseline else oracle_steps_baseline(case, tree, args.steps),
     }
     m.close()
     print(json.dumps(out), flush=True)
@@ -378,8 +406,8 @@ def learned3d(args):
         "learned_speedup_vs_every_4": t_p4 / t_ln,
         "learned_speedup_vs_every_substep": t_ref / t_ln,
         "cadence_sweep": sweep,
-        "roofline": None,
-        "cpu_baseline": None,
+        "roofline": mlp_roofline(fl / ms / 1e9),
+        "cpu_baseline": None if args.no_cpu_baseline else oracle_steps_baseline(case, tree, args.steps),
     }
     m.close()
     print(json.dumps(out), flush=True)
