@@ -1990,9 +1990,14 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) k_
                 sy.gC = C;
             }
         }
+        // the neighbour ids are loaded with the node's own data (for every
+        // valid lane: one round of memory latency fewer) and kept for members
         int32_t nbs[2 * D];
 #pragma unroll
-        for (int f = 0; f < 2 * D; ++f) nbs[f] = member ? d.nbr[f * C + y] : 0;
+        for (int f = 0; f < 2 * D; ++f) {
+            const int32_t nb = valid ? d.nbr[f * C + y] : 0;
+            nbs[f] = member ? nb : 0;
+        }
         // face f's neighbour: ACTI level in bits 4f..4f+3, cluster + 1 in
         // bits 5f..5f+4 (cluster is -1..15); its metadata from its lane when
         // it lies in the window
@@ -2060,7 +2065,8 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) k_
             const bool act = j < qn;
             const int q = act ? Q[j] : lane;
             const int zl = q & 63, dim = (q >> 6) & 3;
-            // the queued cell's state and time from its lane (every lane takes part)
+            // the queued cell's state, time, size, level and - neighbours
+            // from its lane (every lane takes part)
             Side<MODEL, D> sz;
 #pragma unroll
             for (int e = 0; e < NE; ++e) {
@@ -2068,6 +2074,11 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) k_
                 sz.r[e] = __shfl(sy.r[e], zl);
             }
             sz.t = __shfl(sy.t, zl);
+            const double zs = __shfl(ys, zl);
+            const int zlev = __shfl(ylev, zl);
+            int zm[D];
+#pragma unroll
+            for (int a = 0; a < D; ++a) zm[a] = __shfl(nbs[2 * a], zl);
             if (act) {
                 const int kind = (q >> 8) & 7;
                 const int32_t z = (int32_t) (base + zl);
@@ -2075,7 +2086,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) k_
                 if (kind == WK_BOUNDARY) {
                     boundary_flux<MODEL, D>(d, c, z, 2 * dim, dim, 0, dt, F);
                 } else {
-                    const int32_t nb = d.nbr[(2 * dim) * C + z];
+                    const int32_t nb = pick_i<D>(zm, dim);
 #pragma unroll
                     for (int e = 0; e < NE; ++e) sz.s[e] = d.slope[(dim * NE + e) * C + z];
                     if constexpr (MODEL == MULTI) {
@@ -2088,8 +2099,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) k_
                     }
                     Side<MODEL, D> sn;
                     load_side<MODEL, D>(d, nb, dim, sn);
-                    const double zc = d.center[dim * C + z], nc = d.center[dim * C + nb], zs = d.size[z];
-                    const int zlev = d.level[z];
+                    const double zc = d.center[dim * C + z], nc = d.center[dim * C + nb];
                     if (kind == WK_OWN) {
                         face_flux_sides_body<MODEL, D>(c, sz, zc, zs, zlev, sn, nc, dim, 0, dt, F);
                         if ((q >> 11) & 1) {
@@ -2262,7 +2272,9 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) k_
         int qn = 0;
 #pragma unroll
         for (int f = 0; f < 2 * D; ++f) {
-            const int32_t nb = member ? d.nbr[f * C + y] : 0;
+            // loaded for every valid lane, with the node's own data
+            const int32_t nv = valid ? d.nbr[f * C + y] : 0;
+            const int32_t nb = member ? nv : 0;
             nbs[f] = nb;
             const int64_t o = (int64_t) nb - base;
             const bool out = nb != 0 && !(o >= 0 && o < 64);
