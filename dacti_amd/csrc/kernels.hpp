@@ -1945,9 +1945,10 @@ __device__ __forceinline__ void shfl_side(const Side<MODEL, D> &a, int o, Side<M
 // (lane permutes), and so do its metadata word and the + face fluxes; only
 // neighbours outside the window are read from memory.  LDS holds only the
 // compacted pass (queue and results).  Three waves per SIMD: the register
-// budget of a fourth (128 VGPRs) spills and ran 10% slower.
+// budget of a fourth (128 VGPRs) spills and ran 10% slower.  Two for the
+// five-equation model, whose sides spill heavily in 168 VGPRs.
 template <int MODEL, int D>
-__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) k_flux_window(const Item *__restrict__ items, const MConst *__restrict__ mcs) {
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(MODEL == MULTI ? 2 : 3))) k_flux_window(const Item *__restrict__ items, const MConst *__restrict__ mcs) {
     using S = Sz<MODEL, D>;
     constexpr int NF = S::NF, NE = S::NE;
     __shared__ double M[D][NF][64];  // - face flux evaluated in the compacted pass
@@ -2228,7 +2229,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) k_
 // and sweep_decide on the same values, so the results are the same bit for
 // bit.
 template <int MODEL, int D>
-__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) k_sweep_window(const Item *__restrict__ items, const MConst *__restrict__ mcs) {
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(MODEL == MULTI ? 2 : 3))) k_sweep_window(const Item *__restrict__ items, const MConst *__restrict__ mcs) {
     using S = Sz<MODEL, D>;
     constexpr int NE = S::NE, HC = 128;
     __shared__ double P[NE][64];   // states of the window's ids
