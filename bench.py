@@ -543,9 +543,9 @@ def main():
                     help="box: 3D periodic Euler box (default); batched: shock-bubble instances, batch vs plain; "
                          "learned: refinement predictor vs the estimator; drop: 3D shock-drop with AMR; "
                          "bubble: 2D shock-bubble with AMR")
-    ap.add_argument("--graphs", action="store_true", help="bubble: persistent grid and graph replay")
-    ap.add_argument("--grid", type=int, default=0, help="bubble: persistent grid (0: one block per tile)")
-    ap.add_argument("--block", type=int, default=128, help="bubble: threads per list tile with --grid/--graphs")
+    ap.add_argument("--graphs", action="store_true", help="box, bubble: persistent grid and graph replay")
+    ap.add_argument("--grid", type=int, default=0, help="box, bubble: persistent grid (0: one block per tile)")
+    ap.add_argument("--block", type=int, default=128, help="box, bubble: threads per list tile with --grid/--graphs")
     ap.add_argument("--dense", type=int, default=1, choices=(0, 1, 2),
                     help="box: flux op by windows for dense clusters (0 never, 1 auto, 2 always)")
     ap.add_argument("--base", type=int, default=None,
@@ -611,6 +611,9 @@ def main():
     print(f"[bench] tree built: {tree.n} nodes in {time.perf_counter() - t_build:.1f} s", file=sys.stderr, flush=True)
     g = Integrator(case, tree)
     g.set_dense(args.dense)
+    if args.graphs or args.grid:
+        # a fixed grid lets the small substeps run planned on the device
+        g.set_launch(block=args.block, grid=args.grid or 2048, graphs=bool(args.graphs))
     print("[bench] integrator created", file=sys.stderr, flush=True)
     for w in range(args.warmup):
         g.integrate()
@@ -658,7 +661,10 @@ def main():
                                f"({(1 << args.level) ** 3} leaves), LTS on, no refinement",
                    "leaves": (1 << args.level) ** 3, "model": "euler<3>", "limiter": "minmod",
                    "max_cfl": case.max_cfl, "global_dt": s1.global_dt, "max_acti_level": s1.max_acti_level,
-                   "cell_updates_timed": updates, "substeps_timed": launches},
+                   "cell_updates_timed": updates, "substeps_timed": launches,
+                   "launch": {"block": args.block if (args.graphs or args.grid) else 256,
+                              "grid": (args.grid or 2048) if (args.graphs or args.grid) else 0,
+                              "graphs": bool(args.graphs)}},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": pf["bytes"] * updates / max(launches, 1) if pf["bytes"] else None,

@@ -47,16 +47,22 @@ struct BOps {
         k_sel_scan<0><<<1, 1024, 0, st>>>(it, b->d_toffp, n, b->d_tilecnt);
         k_sel_scatter<0><<<gp, 256, 0, st>>>(it, b->d_toffp, n, b->d_tilecnt);
         // the ops over the set-aside cells stride over the list tiles; next to
-        // a window op that set aside few cells a bounded grid does
-        const unsigned gr = b->dense ? std::min(gl, 1024u) : gl;
-        if (b->dense) k_flux_window<MODEL, D><<<window_grid(b), 64, 0, st>>>(it, b->d_mcs);
-        else k_batch<<<gl, B, 0, st>>>(it, b->d_mcs, b->d_toff, n, OpFluxSimple<MODEL, D>{});
-        k_batch<<<gr, B, 0, st>>>(it, b->d_mcs, b->d_toff, n, OpFluxRest<MODEL, D>{});
-        k_batch<<<gr, B, 0, st>>>(it, b->d_mcs, b->d_toff, n, OpDepositCoarse<MODEL, D>{});
+        // a window op (one instance, few cells set aside) they take the
+        // set-aside cells directly on a bounded grid
+        const unsigned gr = std::min(gl, 1024u);
+        if (b->dense) {
+            k_flux_window<MODEL, D><<<window_grid(b), 64, 0, st>>>(it, b->d_mcs);
+            k_rest<<<gr, B, 0, st>>>(it, b->d_mcs, n, OpFluxRest<MODEL, D>{});
+            k_rest<<<gr, B, 0, st>>>(it, b->d_mcs, n, OpDepositCoarse<MODEL, D>{});
+        } else {
+            k_batch<<<gl, B, 0, st>>>(it, b->d_mcs, b->d_toff, n, OpFluxSimple<MODEL, D>{});
+            k_batch<<<gl, B, 0, st>>>(it, b->d_mcs, b->d_toff, n, OpFluxRest<MODEL, D>{});
+            k_batch<<<gl, B, 0, st>>>(it, b->d_mcs, b->d_toff, n, OpDepositCoarse<MODEL, D>{});
+        }
         k_batch<<<gl, B, 0, st>>>(it, b->d_mcs, b->d_toff, n, OpUpdate<MODEL, D>{});
         if (b->dense) {
             k_sweep_window<MODEL, D><<<window_grid(b), 64, 0, st>>>(it, b->d_mcs);
-            k_batch<<<gr, B, 0, st>>>(it, b->d_mcs, b->d_toff, n, OpSweepRest<MODEL, D>{});
+            k_rest<<<gr, B, 0, st>>>(it, b->d_mcs, n, OpSweepRest<MODEL, D>{});
         } else {
             k_batch<<<gl, B, 0, st>>>(it, b->d_mcs, b->d_toff, n, OpSweep<MODEL, D>{0});
         }

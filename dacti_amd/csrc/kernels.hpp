@@ -1882,25 +1882,51 @@ struct OpFluxSimple {
 };
 // The set-aside cells are few and each is a long chain of dependent
 // loads: spread them round-robin over all of the item's tiles (blocks), so
-// that every CU takes some instead of the first few taking all.
+// that every CU takes some instead of the first few taking all.  Launched
+// over the list tiles (k_batch, list index li) or over the set-aside cells
+// themselves (k_rest, r).
 template <int MODEL, int D>
 struct OpFluxRest {
+    __device__ __forceinline__ int count(const Item &it) const { return it.st->n_rest; }
+    __device__ __forceinline__ void cell(const Item &it, const MConst &c, int r) const {
+        flux_cell<MODEL, D>(it.d, c, it.rest[r], it.L, it.dt);
+    }
     __device__ __forceinline__ void operator()(const Item &it, const MConst &c, int li) const {
         const int B = (int) blockDim.x, T = (it.nl + B - 1) / B;
         const int r = (li % B) * T + li / B;
         if (r >= it.st->n_rest) return;
-        flux_cell<MODEL, D>(it.d, c, it.rest[r], it.L, it.dt);
+        cell(it, c, r);
     }
 };
 template <int MODEL, int D>
 struct OpDepositCoarse {
-    __device__ __forceinline__ void operator()(const Item &it, const MConst &, int li) const {
+    __device__ __forceinline__ int count(const Item &it) const { return it.st->n_rest; }
+    __device__ __forceinline__ void cell(const Item &it, const MConst &, int r) const {
+        deposit_coarse<MODEL, D>(it.d, it.rest[r], it.L);
+    }
+    __device__ __forceinline__ void operator()(const Item &it, const MConst &c, int li) const {
         const int B = (int) blockDim.x, T = (it.nl + B - 1) / B;
         const int r = (li % B) * T + li / B;
         if (r >= it.st->n_rest) return;
-        deposit_coarse<MODEL, D>(it.d, it.rest[r], it.L);
+        cell(it, c, r);
     }
 };
+
+// An op over the set-aside cells of each item with a list, r = 0 ..
+// op.count(item) - 1, spread round-robin over the blocks.  For a single
+// instance next to the window ops: the grid does not stride over the list's
+// tiles (a dense cluster has tens of thousands) to find the few cells.
+template <class Op>
+__global__ void __launch_bounds__(256) k_rest(const Item *__restrict__ items, const MConst *__restrict__ mcs,
+                                              int n_items, Op op) {
+    const int G = (int) gridDim.x, B = (int) blockDim.x;
+    for (int i = 0; i < n_items; ++i) {
+        const Item &it = items[i];
+        if (it.nl == 0) continue;
+        const int nr = op.count(it);
+        for (int r = (int) blockIdx.x + G * (int) threadIdx.x; r < nr; r += G * B) op.cell(it, mcs[i], r);
+    }
+}
 
 // ------------------------------------------------------- dense clusters
 // The lean flux op for a cluster that fills most of the pool (a uniform or
@@ -2378,9 +2404,11 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(MODEL =
 
 template <int MODEL, int D>
 struct OpSweepRest {
-    __device__ __forceinline__ void operator()(const Item &it, const MConst &c, int li) const {
-        sweep_body<MODEL, D>(it.d, c, it.rest, it.st->n_rest2, li, it.L, it.dt, it.counter, 0, it.st, it.remesh);
+    __device__ __forceinline__ int count(const Item &it) const { return it.st->n_rest2; }
+    __device__ __forceinline__ void cell(const Item &it, const MConst &c, int r) const {
+        sweep_body<MODEL, D>(it.d, c, it.rest, it.st->n_rest2, r, it.L, it.dt, it.counter, 0, it.st, it.remesh);
     }
+    __device__ __forceinline__ void operator()(const Item &it, const MConst &c, int li) const { cell(it, c, li); }
 };
 
 template <int MODEL, int D>
@@ -2795,51 +2823,85 @@ static __global__ void k_zero_hist(const Item *__restrict__ items, int n_items) 
     if (i < n_items && threadIdx.x < 16) items[i].hist[threadIdx.x] = 0;
 }
 
+// the number of bytes of v equal to b (exact zero-byte test on v ^ b..b)
+__device__ __forceinline__ unsigned count_bytes_eq(uint64_t v, uint8_t b) {
+    const uint64_t x = v ^ (0x0101010101010101ull * b);
+    const uint64_t t = ~(((x & 0x7f7f7f7f7f7f7f7full) + 0x7f7f7f7f7f7f7f7full) | x | 0x7f7f7f7f7f7f7f7full);
+    return (unsigned) __popcll(t);
+}
+
+// Per-level counts of the listed cells' clusters over the pool: a streaming
+// pass.  Each thread keeps its own 16 level counts over the tiles its block
+// takes (8 ids per tile, four tiles' loads in flight); the wave adds them up
+// and the block's counts go to the item's global counters when the item
+// changes and at the end (few global atomics).
 static __global__ void __launch_bounds__(256) k_pool_hist(const Item *__restrict__ items, const int *__restrict__ toff,
                                                           int n_items) {
-    // counts gather in LDS over all tiles a block takes and go to the
-    // item's global counters when the item changes (few global atomics)
+    constexpr int U = 4;
     __shared__ unsigned int h[16];
     const int total = toff[n_items];
+    if ((int) blockIdx.x >= total) return;  // no tile for this block
+    const int lane = (int) threadIdx.x & 63;
     int cur = -1;
+    unsigned cnt[16];
+#pragma unroll
+    for (int l = 0; l < 16; ++l) cnt[l] = 0;
     if (threadIdx.x < 16) h[threadIdx.x] = 0;
     __syncthreads();
-    for (int t = blockIdx.x; t < total; t += gridDim.x) {
-        const int tile = xcd_block(t, total);
-        const int i = find_item(toff, n_items, tile);
-        if (i != cur) {
-            __syncthreads();  // every wave has counted the previous tile
-            if (cur >= 0 && threadIdx.x < 16 && h[threadIdx.x])
-                atomicAdd(&items[cur].hist[threadIdx.x], (unsigned long long) h[threadIdx.x]);
-            __syncthreads();
-            if (threadIdx.x < 16) h[threadIdx.x] = 0;
-            __syncthreads();
-            cur = i;
-        }
-        const Item &it = items[i];
-        const int64_t base8 = (int64_t) (tile - toff[i]) * POOL_TILE + 8 * (int64_t) threadIdx.x;
-        // clusters of the thread's 8 ids (absent ids and id 0 as -1)
-        uint64_t v = load8(it.d.cluster, base8, it.n, 0xff);
-        if (base8 == 0) v |= 0xff;
-        unsigned present = 0;
+    // the thread counts into the block's, the block's into item cur's
+    auto flush = [&]() {
+        // the levels counted in the wave, then one sum per level
+        unsigned pm = 0;
 #pragma unroll
-        for (int j = 0; j < 8; ++j) {
-            const unsigned cl = (unsigned) ((v >> (8 * j)) & 0xff);
-            if (cl < 16) present |= 1u << cl;
+        for (int l = 0; l < 16; ++l) pm |= (unsigned) (cnt[l] != 0) << l;
+        for (int o = 32; o > 0; o >>= 1) pm |= (unsigned) __shfl_xor((int) pm, o);
+#pragma unroll
+        for (int l = 0; l < 16; ++l) {
+            if ((pm >> l) & 1) {
+                unsigned c = cnt[l];
+                for (int o = 32; o > 0; o >>= 1) c += (unsigned) __shfl_xor((int) c, o);
+                if (lane == 0) atomicAdd(&h[l], c);
+            }
+            cnt[l] = 0;
         }
-        // the levels present in the wave, then one count per level
-        for (int o = 32; o > 0; o >>= 1) present |= (unsigned) __shfl_xor((int) present, o);
-        while (present) {
-            const int l = __ffs(present) - 1;
-            present &= present - 1;
-            int c = __popc(bytes_eq(v, (uint8_t) l));
-            for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o);
-            if ((threadIdx.x & 63) == 0) atomicAdd(&h[l], (unsigned) c);
+        __syncthreads();
+        if (cur >= 0 && threadIdx.x < 16 && h[threadIdx.x])
+            atomicAdd(&items[cur].hist[threadIdx.x], (unsigned long long) h[threadIdx.x]);
+        __syncthreads();
+        if (threadIdx.x < 16) h[threadIdx.x] = 0;
+        __syncthreads();
+    };
+    for (int t0 = (int) blockIdx.x; t0 < total; t0 += U * (int) gridDim.x) {
+        uint64_t v[U];
+        int ti[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int t = t0 + u * (int) gridDim.x;
+            ti[u] = -1;
+            v[u] = ~0ull;
+            if (t < total) {
+                const int tile = xcd_block(t, total);
+                const int i = find_item(toff, n_items, tile);
+                const Item &it = items[i];
+                const int64_t base8 = (int64_t) (tile - toff[i]) * POOL_TILE + 8 * (int64_t) threadIdx.x;
+                // clusters of the thread's 8 ids (absent ids and id 0 as -1)
+                v[u] = load8(it.d.cluster, base8, it.n, 0xff);
+                if (base8 == 0) v[u] |= 0xff;
+                ti[u] = i;
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            if (ti[u] < 0) break;
+            if (ti[u] != cur) {
+                if (cur >= 0) flush();
+                cur = ti[u];
+            }
+#pragma unroll
+            for (int l = 0; l < 16; ++l) cnt[l] += count_bytes_eq(v[u], (uint8_t) l);
         }
     }
-    __syncthreads();
-    if (cur >= 0 && threadIdx.x < 16 && h[threadIdx.x])
-        atomicAdd(&items[cur].hist[threadIdx.x], (unsigned long long) h[threadIdx.x]);
+    flush();
 }
 
 // ------------------------------------------------- batched remesh phases
