@@ -1123,14 +1123,13 @@ __device__ __forceinline__ void sweep_side(const Dev &d, const MConst &c, int32_
 template <int MODEL, int D>
 __device__ __forceinline__ void sweep_decide(const Dev &d, const MConst &c, int32_t y, const double *pc, const double *ctr,
                                              double ys, int ylev, int L, double dt, int counter, bool deferred, double err,
-                                             Status *st, int remesh) {
+                                             Status *st, int remesh, uint8_t fl) {
     bool lowered = false, marked = false;
     if (deferred) {
         d.mark[y] = MK_DEFER;
     } else {
         err = sqrt(err);
         d.error[y] = err;
-        uint8_t fl = d.flags[y];
         if (fl & F_CAN_SPLIT) {
             if (!scene_can_split<D>(c, ctr, ylev)) fl &= ~F_CAN_SPLIT;
             d.flags[y] = fl;
@@ -1252,7 +1251,7 @@ __device__ __forceinline__ void sweep_body(const Dev &d, const MConst &c, const 
     }
     if (pass == 0)
         for (int e = 0; e < NE; ++e) __builtin_nontemporal_store(rhs[e], &d.rhs[e * C + y]);
-    sweep_decide<MODEL, D>(d, c, y, pc, ctr, ys, ylev, L, dt, counter, deferred, err, st, remesh);
+    sweep_decide<MODEL, D>(d, c, y, pc, ctr, ys, ylev, L, dt, counter, deferred, err, st, remesh, d.flags[y]);
 }
 
 // apply the stored LTS decisions (impl/adaptive_refinement.hpp:308-321) and
@@ -2294,6 +2293,13 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(MODEL =
             if constexpr (S::NG > 0) T[lane] = d.t[y];
         }
         FLV[lane] = flv;
+        // the member's own size and flags, in flight during the halo phase
+        double ys = 0.0;
+        uint8_t yfl = 0;
+        if (member) {
+            ys = d.size[y];
+            yfl = d.flags[y];
+        }
         int32_t nbs[2 * D];
         int hs[2 * D];
         int qn = 0;
@@ -2342,7 +2348,6 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(MODEL =
                 // that may still split
 #pragma unroll
                 for (int a = 0; a < D; ++a) ctr[a] = (flv & F_CAN_SPLIT) ? d.center[a * C + y] : 0.0;
-                const double ys = d.size[y];
                 const double yt = S::NG > 0 ? T[lane] : 0.0;  // the time enters the gradients only
                 const double kap = MODEL == MULTI ? d.k[S::IKAPPA * C + y] : 0.0;
                 double a_ip_in, a_in_ip, a_in_ia;
@@ -2395,7 +2400,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(MODEL =
                 }
 #pragma unroll
                 for (int e = 0; e < NE; ++e) __builtin_nontemporal_store(rhs[e], &d.rhs[e * C + y]);
-                sweep_decide<MODEL, D>(d, c, y, pc, ctr, ys, ylev, L, it.dt, it.counter, false, err, it.st, it.remesh);
+                sweep_decide<MODEL, D>(d, c, y, pc, ctr, ys, ylev, L, it.dt, it.counter, false, err, it.st, it.remesh, yfl);
             }
         }
         __syncthreads();
