@@ -831,8 +831,10 @@ __device__ __forceinline__ void flux_simple_body(const Dev &d, const MConst &c, 
                     d.acc[(fm * NF + e) * C + y] = 0.0;
                 }
             }
+            // nbm is a same-level leaf (y is simple), so its + neighbour is y
+            // and it evaluates the face when it updates in this substep
             const bool y_computes = nacti < L;
-            const bool nb_computes = !y_computes && ncl == L && (handed || d.nbr[fp * C + nbm] == y);
+            const bool nb_computes = !y_computes && ncl == L;
             if (nb_computes && handed) {
                 // the same evaluation (owner nbm, its + face) made by nbm's lane
 #pragma unroll
@@ -2062,8 +2064,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(MODEL =
             const int64_t o64 = (int64_t) nbm - base;
             const bool inr = nbm != 0 && o64 >= 0 && o64 < 64;
             const int o = inr ? (int) o64 : lane;
-            // the window neighbour's + neighbour and + face decisions (every lane takes part)
-            const int32_t nxt_w = __shfl(nbs[f + 1], o);
+            // the window neighbour's + face decisions (every lane takes part)
             const int cp_w = __shfl(cp, o);
             int kind = WK_NONE, dep = 0;
             if (part) {
@@ -2074,9 +2075,10 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(MODEL =
                     kind = WK_OWN;
                     dep = ncl != L;
                 } else if (ncl == L) {
+                    // nbm is a same-level leaf (y takes part), so its + neighbour
+                    // is y: it evaluates the face
                     const bool inwin = inr && ((pm >> o) & 1);
-                    const int32_t nxt = inwin ? nxt_w : d.nbr[(f + 1) * C + nbm];
-                    if (nxt == y) kind = inwin && ((cp_w >> dim) & 1) ? WK_SHARED : WK_NEIGHBOUR;
+                    kind = inwin && ((cp_w >> dim) & 1) ? WK_SHARED : WK_NEIGHBOUR;
                 }
             }
             kinds |= kind << (3 * dim);
