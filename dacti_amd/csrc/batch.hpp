@@ -373,6 +373,9 @@ struct BOps {
         b->h_toff[n] = (int) tl;
         int rc = 0;
         bool remeshed = false;
+        bool fixed = any_list;
+        for (int i = 0; i < n && fixed; ++i) fixed = b->hs[(size_t) i]->static_mesh;
+        if (fixed) return substep_static(b, L, part, changed, force_break);
         if (any_list && !(n == 1 && b->hs[0]->dense == 2) && device_step(b, part, L, rc))
             return substep_device(b, L, part, changed, force_break);
         if (rc) return rc;
@@ -515,6 +518,71 @@ struct BOps {
                 continue;
             }
             h->n = s.n_pool;
+        }
+        return finish_substep(b, L, part, changed);
+    }
+
+    // A substep of meshes that cannot change (every instance on one tree
+    // level with no cell that may split or unsplit): no cell is deferred
+    // (that needs a coarser neighbour) or marked, so head and tail run as
+    // one sequence with one wait, on the launch grids of the host-planned
+    // path (the window ops for a dense cluster included).
+    static int launch_static(dacti_batch *b) {
+        const int rc = launch_head_ops(b, false);
+        if (rc) return rc;
+        return launch_tail(b);
+    }
+
+    static int substep_static(dacti_batch *b, int L, std::vector<char> &part, std::vector<char> &changed,
+                              int *force_break) {
+        const int n = b->n;
+        int rc = upload_tables(b, changed);
+        if (rc) return rc;
+        b->dense = false;
+        if (n == 1 && b->h_items[0].nl > 0) {
+            const dacti_handle *h0 = b->hs[0];
+            b->dense = h0->dense == 2 || (h0->dense == 1 && 2 * (int64_t) b->h_items[0].nl >= h0->n);
+        }
+        if (b->timing && hipEventRecord(b->ev[0], b->stream) != hipSuccess) return DACTI_E_HIP;
+        rc = run_seq(b, b->dense ? &b->g_static_dense : &b->g_static, launch_static);
+        if (rc) return rc;
+        if (b->timing && hipEventRecord(b->ev[1], b->stream) != hipSuccess) return DACTI_E_HIP;
+        rc = sync(b);
+        if (rc) return rc;
+        if (b->timing) {
+            float a = 0;
+            if (hipEventElapsedTime(&a, b->ev[0], b->ev[1]) != hipSuccess) return DACTI_E_HIP;
+            b->flux_ms += a;
+            b->substep_ms += a;
+        }
+        b->launches++;
+        for (int i = 0; i < n; ++i) {
+            Item &it = b->h_items[i];
+            if (it.nl == 0) continue;
+            dacti_handle *h = b->hs[(size_t) i];
+            const Status &s = b->h_st[i];
+            h->flux_launches++;
+            h->cell_updates += it.nl;
+            b->cell_updates += it.nl;
+            if (s.n_list != it.nl) {
+                b->err = "internal: cluster list size differs from the level histogram";
+                return DACTI_E_STATE;
+            }
+            if (s.nan_flag) {
+                // this instance stops here (its sweep and LTS apply were
+                // skipped on the device, its level histogram is not taken)
+                *force_break = 1;
+                b->broken[(size_t) i] = 1;
+                part[(size_t) i] = 0;
+                changed[(size_t) i] = 0;
+                it.nl = 0;
+                it.flush = 0;
+                continue;
+            }
+            if (s.n_deferred > 0 || s.n_marked > 0) {
+                b->err = "internal: cells deferred or marked in a mesh that cannot change";
+                return DACTI_E_STATE;
+            }
         }
         return finish_substep(b, L, part, changed);
     }

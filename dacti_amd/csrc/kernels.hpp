@@ -2660,7 +2660,8 @@ static __global__ void __launch_bounds__(256) k_pool_flush(const Item *__restric
         const int tile = xcd_block(t, total);
         const int i = find_item(toff, n_items, tile);
         const Item &it = items[i];
-        if (!it.flush) continue;
+        // an instance whose state turned NaN stops before the flush
+        if (!it.flush || it.st->nan_flag) continue;
         // 8 ids per thread; only ids whose buffer holds level L change
         const int64_t base8 = (int64_t) (tile - toff[i]) * POOL_TILE + 8 * (int64_t) threadIdx.x;
         const unsigned m = bytes_eq(load8(it.d.buffer, base8, it.n, 0xff), (uint8_t) it.L);

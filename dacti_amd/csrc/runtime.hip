@@ -246,7 +246,9 @@ void batch_drop_graphs(dacti_batch *b) {
     if (b->g_tail) (void) hipGraphExecDestroy(b->g_tail);
     if (b->g_head_dense) (void) hipGraphExecDestroy(b->g_head_dense);
     if (b->g_step) (void) hipGraphExecDestroy(b->g_step);
-    b->g_head = b->g_tail = b->g_head_dense = b->g_step = nullptr;
+    if (b->g_static) (void) hipGraphExecDestroy(b->g_static);
+    if (b->g_static_dense) (void) hipGraphExecDestroy(b->g_static_dense);
+    b->g_head = b->g_tail = b->g_head_dense = b->g_step = b->g_static = b->g_static_dense = nullptr;
 }
 
 void batch_free(dacti_batch *b) {
@@ -468,6 +470,24 @@ int dacti_create(const dacti_case *c, const dacti_tree *t, dacti_handle **out) {
         for (int64_t i = 1; i < t->n_nodes; ++i)
             if (t->tree_level[i] > lmax) lmax = t->tree_level[i], first = i;
         h->walign = (int) (first & 63);
+        // a mesh that cannot change: every active leaf on one tree level, at
+        // or above every level the case refines to (Scene::can_split,
+        // scene/scene.hpp:50-61, is false for all of them), and none that
+        // may unsplit or is flagged for a split.  Splits and unsplits are
+        // the only way leaves gain those flags or another level.
+        bool fixed = true;
+        int lev = -1;
+        for (int64_t i = 1; i < t->n_nodes && fixed; ++i) {
+            const uint8_t f = t->flags[i];
+            if (!(f & F_LEAF) || !(f & F_ACTIVE) || (f & F_GARBAGE)) continue;
+            if (f & (F_CAN_UNSPLIT | F_SHOULD_SPLIT)) fixed = false;
+            if (lev < 0) lev = t->tree_level[i];
+            else if (t->tree_level[i] != lev) fixed = false;
+        }
+        if (lev < c->max_refinement_level) fixed = false;
+        for (int z = 0; z < c->n_zones; ++z)
+            if (lev < c->zone_level[z]) fixed = false;
+        h->static_mesh = fixed;
     }
     h->model = c->model;
     h->D = c->dim;
@@ -853,6 +873,7 @@ int dacti_set_remesh(dacti_handle *h, int32_t on) {
 int dacti_mark_split(dacti_handle *h, const int32_t *ids, int64_t n) {
     if (!h || (n > 0 && !ids) || n < 0) return DACTI_E_ARG;
     if (n == 0) return DACTI_OK;
+    h->static_mesh = false;  // flagged leaves split at the next remesh
     int32_t *d = nullptr;
     HIPCHK(h, hipMalloc(&d, sizeof(int32_t) * (size_t) n));
     hipError_t e = hipMemcpy(d, ids, sizeof(int32_t) * (size_t) n, hipMemcpyHostToDevice);
@@ -892,6 +913,7 @@ int dacti_locate_levels(dacti_handle *h, const double *pts, int64_t n, int8_t *l
 int dacti_mark_predicted(dacti_handle *h, const float *logits, const int32_t *ids, int64_t n, int32_t rings,
                          int64_t *n_marked) {
     if (!h || n < 0 || rings < 0 || !n_marked || (n > 0 && (!logits || !ids))) return DACTI_E_ARG;
+    h->static_mesh = false;  // flagged leaves split at the next remesh
     int rc = DACTI_E_ARG;
 #define DACTI_CASE_MARK(M, Dm) \
     if (h->model == M && h->D == Dm) rc = variant_mark_predicted_##M##_##Dm(h, logits, ids, n, rings, n_marked);

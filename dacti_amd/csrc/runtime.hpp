@@ -95,6 +95,7 @@ struct dacti_handle {
     int remesh = 1;  // 0: refinement marks paused (host-set cadence)
     int dense = 1;   // window flux op for dense clusters: 0 never, 1 when the cluster fills half the pool, 2 always
     int walign = 0;  // window alignment: first node id of the initial tree's deepest level, mod 64
+    bool static_mesh = false;  // one tree level, no cell may split or unsplit: the mesh never changes
     dacti_substep_hook hook = nullptr;  // called after every substep (impl/integrate.hpp:64)
     void *hook_user = nullptr;
     std::string err;
@@ -129,6 +130,8 @@ struct dacti_batch {
     hipGraphExec_t g_head = nullptr, g_tail = nullptr;
     hipGraphExec_t g_head_dense = nullptr;  // head with the window flux op (a dense cluster of a batch of one)
     hipGraphExec_t g_step = nullptr;        // a whole substep planned on the device
+    hipGraphExec_t g_static = nullptr;      // head + tail of meshes that cannot change
+    hipGraphExec_t g_static_dense = nullptr;
     bool dense = false;                     // this substep takes the window flux op
     bool timing = false;
     // head start/end, tail start/end, remesh start/end
