@@ -49,20 +49,27 @@ struct BOps {
         // the ops over the set-aside cells stride over the list tiles; next to
         // a window op (one instance, few cells set aside) they take the
         // set-aside cells directly on a bounded grid
+        // (a mesh that cannot change has one tree level: nothing is set aside
+        // and nothing marked, which the host checks in the status)
         const unsigned gr = std::min(gl, 1024u);
+        const bool fixed = b->fixed_mesh;
         if (b->dense) {
             k_flux_window<MODEL, D><<<window_grid(b), 64, 0, st>>>(it, b->d_mcs);
-            k_rest<<<gr, B, 0, st>>>(it, b->d_mcs, n, OpFluxRest<MODEL, D>{});
-            k_rest<<<gr, B, 0, st>>>(it, b->d_mcs, n, OpDepositCoarse<MODEL, D>{});
+            if (!fixed) {
+                k_rest<<<gr, B, 0, st>>>(it, b->d_mcs, n, OpFluxRest<MODEL, D>{});
+                k_rest<<<gr, B, 0, st>>>(it, b->d_mcs, n, OpDepositCoarse<MODEL, D>{});
+            }
         } else {
             k_batch<<<gl, B, 0, st>>>(it, b->d_mcs, b->d_toff, n, OpFluxSimple<MODEL, D>{});
-            k_batch<<<gl, B, 0, st>>>(it, b->d_mcs, b->d_toff, n, OpFluxRest<MODEL, D>{});
-            k_batch<<<gl, B, 0, st>>>(it, b->d_mcs, b->d_toff, n, OpDepositCoarse<MODEL, D>{});
+            if (!fixed) {
+                k_batch<<<gl, B, 0, st>>>(it, b->d_mcs, b->d_toff, n, OpFluxRest<MODEL, D>{});
+                k_batch<<<gl, B, 0, st>>>(it, b->d_mcs, b->d_toff, n, OpDepositCoarse<MODEL, D>{});
+            }
         }
         k_batch<<<gl, B, 0, st>>>(it, b->d_mcs, b->d_toff, n, OpUpdate<MODEL, D>{});
         if (b->dense) {
             k_sweep_window<MODEL, D><<<window_grid(b), 64, 0, st>>>(it, b->d_mcs);
-            k_rest<<<gr, B, 0, st>>>(it, b->d_mcs, n, OpSweepRest<MODEL, D>{});
+            if (!fixed) k_rest<<<gr, B, 0, st>>>(it, b->d_mcs, n, OpSweepRest<MODEL, D>{});
         } else {
             k_batch<<<gl, B, 0, st>>>(it, b->d_mcs, b->d_toff, n, OpSweep<MODEL, D>{0});
         }
@@ -77,7 +84,7 @@ struct BOps {
             k_batch<<<gl, B, 0, st>>>(it, b->d_mcs, b->d_toff, n, OpStoreK{S::IKAPPA});
         }
         // split readiness and unsplit detection of the marked cells
-        k_batch<<<gl, B, 0, st>>>(it, b->d_mcs, b->d_toff, n, OpMark2<D>{});
+        if (!fixed) k_batch<<<gl, B, 0, st>>>(it, b->d_mcs, b->d_toff, n, OpMark2<D>{});
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) {
             b->err = std::string("head launch: ") + hipGetErrorString(e);
@@ -375,6 +382,7 @@ struct BOps {
         bool remeshed = false;
         bool fixed = any_list;
         for (int i = 0; i < n && fixed; ++i) fixed = b->hs[(size_t) i]->static_mesh;
+        b->fixed_mesh = fixed;
         if (fixed) return substep_static(b, L, part, changed, force_break);
         if (any_list && !(n == 1 && b->hs[0]->dense == 2) && device_step(b, part, L, rc))
             return substep_device(b, L, part, changed, force_break);
@@ -579,8 +587,8 @@ struct BOps {
                 it.flush = 0;
                 continue;
             }
-            if (s.n_deferred > 0 || s.n_marked > 0) {
-                b->err = "internal: cells deferred or marked in a mesh that cannot change";
+            if (s.n_deferred > 0 || s.n_marked > 0 || s.n_rest > 0 || s.n_rest2 > 0) {
+                b->err = "internal: cells set aside, deferred or marked in a mesh that cannot change";
                 return DACTI_E_STATE;
             }
         }

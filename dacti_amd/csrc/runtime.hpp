@@ -131,6 +131,7 @@ struct dacti_batch {
     hipGraphExec_t g_head_dense = nullptr;  // head with the window flux op (a dense cluster of a batch of one)
     hipGraphExec_t g_step = nullptr;        // a whole substep planned on the device
     hipGraphExec_t g_static = nullptr;      // head + tail of meshes that cannot change
+    bool fixed_mesh = false;                // the substep being launched is one of those
     hipGraphExec_t g_static_dense = nullptr;
     bool dense = false;                     // this substep takes the window flux op
     bool timing = false;
