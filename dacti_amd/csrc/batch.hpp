@@ -99,14 +99,18 @@ struct BOps {
         return DACTI_OK;
     }
 
-    static int launch_tail(dacti_batch *b) {
+    static int launch_tail(dacti_batch *b) { return launch_tail_ops(b, true); }
+
+    // zero: clear the level histograms here (k_sel_scan<0> of a head in the
+    // same sequence clears them otherwise)
+    static int launch_tail_ops(dacti_batch *b, bool zero) {
         const hipStream_t st = b->stream;
         const unsigned gl = list_grid(b), gp = pool_grid(b);
         const int n = b->n;
         Item *it = b->d_items;
         k_batch<<<gl, b->block, 0, st>>>(it, b->d_mcs, b->d_toff, n, OpApply{});
         k_pool_flush<<<gp, 256, 0, st>>>(it, b->d_toffp, n);
-        k_zero_hist<<<(unsigned) n, 64, 0, st>>>(it, n);
+        if (zero) k_zero_hist<<<(unsigned) n, 64, 0, st>>>(it, n);
         k_pool_hist<<<std::min(gp, 1024u), 256, 0, st>>>(it, b->d_toffp, n);
         hipError_t e = hipGetLastError();
         // status and level histograms: one block, one copy
@@ -150,7 +154,7 @@ struct BOps {
             b->err = std::string("remesh launch: ") + hipGetErrorString(e);
             return DACTI_E_HIP;
         }
-        return launch_tail(b);
+        return launch_tail_ops(b, false);
     }
 
     // Whether a substep can run as launch_step: fixed grids and each
@@ -538,7 +542,7 @@ struct BOps {
     static int launch_static(dacti_batch *b) {
         const int rc = launch_head_ops(b, false);
         if (rc) return rc;
-        return launch_tail(b);
+        return launch_tail_ops(b, false);
     }
 
     static int substep_static(dacti_batch *b, int L, std::vector<char> &part, std::vector<char> &changed,

@@ -2593,17 +2593,40 @@ __global__ void __launch_bounds__(256) k_sel_count(const Item *__restrict__ item
 template <int OUT>
 __global__ void __launch_bounds__(1024) k_sel_scan(const Item *__restrict__ items, const int *__restrict__ toff,
                                                    int n_items, int *tilecnt) {
+    constexpr int K = 16;  // counts per thread in one pass
     __shared__ int lds[32];
     for (int i = 0; i < n_items; ++i) {
         const int a = toff[i], b = toff[i + 1];
         int running = 0;
-        for (int s = a; s < b; s += 1024) {
-            const int idx = s + threadIdx.x;
-            const int v = idx < b ? tilecnt[idx] : 0;
+        if (b - a <= 1024 * K) {
+            // one pass: each thread loads its run of consecutive counts at
+            // once, one block scan of the run sums
+            const int per = (b - a + 1023) / 1024;
+            const int s0 = a + (int) threadIdx.x * per;
+            int v[K], sum = 0;
+#pragma unroll
+            for (int k = 0; k < K; ++k) {
+                v[k] = k < per && s0 + k < b ? tilecnt[s0 + k] : 0;
+                sum += v[k];
+            }
             int tot = 0;
-            const int ex = block_exclusive_scan_1024(v, lds, tot);
-            if (idx < b) tilecnt[idx] = running + ex;
-            running += tot;
+            int r = block_exclusive_scan_1024(sum, lds, tot);
+#pragma unroll
+            for (int k = 0; k < K; ++k)
+                if (k < per && s0 + k < b) {
+                    tilecnt[s0 + k] = r;
+                    r += v[k];
+                }
+            running = tot;
+        } else {
+            for (int s = a; s < b; s += 1024) {
+                const int idx = s + threadIdx.x;
+                const int v = idx < b ? tilecnt[idx] : 0;
+                int tot = 0;
+                const int ex = block_exclusive_scan_1024(v, lds, tot);
+                if (idx < b) tilecnt[idx] = running + ex;
+                running += tot;
+            }
         }
         if (threadIdx.x == 0) {
             Status *st = items[i].st;
@@ -2614,6 +2637,8 @@ __global__ void __launch_bounds__(1024) k_sel_scan(const Item *__restrict__ item
                 st->n_list2 = running;
             }
         }
+        // the level histogram the substep's tail gathers starts at zero
+        if (OUT == 0 && threadIdx.x < 16) items[i].hist[threadIdx.x] = 0;
     }
 }
 
