@@ -60,7 +60,9 @@ struct BOps {
                 k_rest<<<gr, B, 0, st>>>(it, b->d_mcs, n, OpDepositCoarse<MODEL, D>{});
             }
         } else {
-            k_batch<<<gl, B, 0, st>>>(it, b->d_mcs, b->d_toff, n, OpFluxSimple<MODEL, D>{});
+            // 64 listed cells a block (the block size is 64, 128 or 256)
+            const unsigned ga = b->grid > 0 ? (unsigned) b->grid : std::max(1u, (unsigned) b->h_toff[n] * (unsigned) (B / 64));
+            k_flux_axes<MODEL, D><<<ga, 64 * D, 0, st>>>(it, b->d_mcs, b->d_toff, n, B);
             if (!fixed) {
                 k_batch<<<gl, B, 0, st>>>(it, b->d_mcs, b->d_toff, n, OpFluxRest<MODEL, D>{});
                 k_batch<<<gl, B, 0, st>>>(it, b->d_mcs, b->d_toff, n, OpDepositCoarse<MODEL, D>{});
