@@ -37,6 +37,14 @@ struct BOps {
 
     static int launch_head(dacti_batch *b) { return launch_head_ops(b, true); }
 
+    // the sweep with a cell's axes on separate wavefronts (k_sweep_axes) for
+    // lists of up to 1 M cells and on fixed grids (a captured sequence takes
+    // one form for every list); one thread per cell for longer lists, which
+    // fill the GPU without it (measured 2.5% faster on the 3D shock-drop)
+    static bool sweep_axes(const dacti_batch *b) {
+        return b->grid > 0 || (int64_t) b->h_toff[b->n] * b->block < (int64_t(1) << 20);
+    }
+
     static int launch_head_ops(dacti_batch *b, bool copy_status) {
         const hipStream_t st = b->stream;
         const unsigned gl = list_grid(b), gp = pool_grid(b);
@@ -73,7 +81,12 @@ struct BOps {
             k_sweep_window<MODEL, D><<<window_grid(b), 64, 0, st>>>(it, b->d_mcs);
             if (!fixed) k_rest<<<gr, B, 0, st>>>(it, b->d_mcs, n, OpSweepRest<MODEL, D>{});
         } else {
-            k_batch<<<gl, B, 0, st>>>(it, b->d_mcs, b->d_toff, n, OpSweep<MODEL, D>{0});
+            if (sweep_axes(b)) {
+                const unsigned ga = b->grid > 0 ? (unsigned) b->grid : std::max(1u, (unsigned) b->h_toff[n] * (unsigned) (B / 64));
+                k_sweep_axes<MODEL, D><<<ga, 64 * D, 0, st>>>(it, b->d_mcs, b->d_toff, n, B, 0);
+            } else {
+                k_batch<<<gl, B, 0, st>>>(it, b->d_mcs, b->d_toff, n, OpSweep<MODEL, D>{0});
+            }
         }
         if constexpr (MODEL == MULTI) {
             for (int r = 0; r < 3; ++r) {
@@ -136,7 +149,7 @@ struct BOps {
         Item *it = b->d_items;
         int rc = launch_head_ops(b, false);
         if (rc) return rc;
-        k_batch<<<G, B, 0, st>>>(it, b->d_mcs, b->d_toff, n, OpSweep<MODEL, D>{1});
+        k_sweep_axes<MODEL, D><<<G, 64 * D, 0, st>>>(it, b->d_mcs, b->d_toff, n, B, 1);
         k_clear_split_counts<true><<<(unsigned) ((n + 63) / 64), 64, 0, st>>>(it, n);
         k_batch<<<G, B, 0, st>>>(it, b->d_mcs, b->d_toff, n, OpMark2<D, true>{});
         k_remesh_plan<<<1, 64, 0, st>>>(it, n, S::NCH, B, b->d_toffr, b->d_toff2, b->d_toffp);
@@ -444,7 +457,13 @@ struct BOps {
                 if (rc) return rc;
             }
             if (deferred) {
-                k_batch<<<list_grid(b), b->block, 0, b->stream>>>(b->d_items, b->d_mcs, b->d_toff, n, OpSweep<MODEL, D>{1});
+                if (sweep_axes(b)) {
+                    const unsigned ga = b->grid > 0 ? (unsigned) b->grid
+                                                    : std::max(1u, (unsigned) b->h_toff[n] * (unsigned) (b->block / 64));
+                    k_sweep_axes<MODEL, D><<<ga, 64 * D, 0, b->stream>>>(b->d_items, b->d_mcs, b->d_toff, n, b->block, 1);
+                } else {
+                    k_batch<<<list_grid(b), b->block, 0, b->stream>>>(b->d_items, b->d_mcs, b->d_toff, n, OpSweep<MODEL, D>{1});
+                }
                 k_clear_split_counts<<<(unsigned) ((n + 63) / 64), 64, 0, b->stream>>>(b->d_items, n);
                 k_batch<<<list_grid(b), b->block, 0, b->stream>>>(b->d_items, b->d_mcs, b->d_toff, n, OpMark2<D>{});
                 if (hipGetLastError() != hipSuccess ||

@@ -968,17 +968,19 @@ __device__ __forceinline__ void sweep_decide(const Dev &d, const MConst &c, int3
 // false at a domain face): the limited slope (impl/slopes.hpp:15-76), the
 // rhs terms of this axis, the gradient and the Laplacian error term
 // (error_estimators.hpp:19-37).
+// sweep_axis_core: the limited slope sl (stored with the gradient in pass
+// 0) and the axis's Laplacian error term sq; sweep_axis adds the axis's rhs
+// terms (pass 0) and sq to the cell's sums
 template <int MODEL, int D>
-__device__ __forceinline__ void sweep_axis(const Dev &d, const MConst &c, int32_t y, int dim, int pass, const double *pc,
-                                           const double *pn, const double *pp, const double *qn, const double *qp,
-                                           double xn, double xp, double tn, double tp, bool hn, bool hp,
-                                           const double *ts, double kap, double a_ip_in, double a_in_ip, double a_in_ia,
-                                           double *rhs, double &err) {
+__device__ __forceinline__ void sweep_axis_core(const Dev &d, const MConst &c, int32_t y, int dim, int pass,
+                                                const double *pc, const double *pn, const double *pp, const double *qn,
+                                                const double *qp, double xn, double xp, double tn, double tp, bool hn,
+                                                bool hp, const double *ts, double *sl, double &sq_out) {
     using S = Sz<MODEL, D>;
     constexpr int NE = S::NE;
     const int64_t C = d.C;
     const double xc = 0.0;
-    double sl[NE], sn[NE], sp[NE];
+    double sn[NE], sp[NE];
     double rn, rp;
     const double dn = xc - xn, dp = xp - xc;
     if (pow2_recip(dn, rn) && pow2_recip(dp, rp)) {
@@ -994,8 +996,6 @@ __device__ __forceinline__ void sweep_axis(const Dev &d, const MConst &c, int32_
     }
     for (int e = 0; e < NE; ++e) sl[e] = limiter(c.limiter, sn[e], sp[e]);
     if (pass == 0) {
-        // rhs[i] gains the dim-j terms in the reference's (i, j, k) order
-        rhs_add_dim<MODEL, D>(c, pc, sl, dim, kap, a_ip_in, a_in_ip, a_in_ia, rhs);
         for (int e = 0; e < NE; ++e) __builtin_nontemporal_store(sl[e], &d.slope[(dim * NE + e) * C + y]);
         if constexpr (S::NG > 0) {
             double gp[S::NG], gn[S::NG];
@@ -1021,6 +1021,19 @@ __device__ __forceinline__ void sweep_axis(const Dev &d, const MConst &c, int32_
         const double v = d2 * c.weights[e];
         sq += v * v;
     }
+    sq_out = sq;
+}
+
+template <int MODEL, int D>
+__device__ __forceinline__ void sweep_axis(const Dev &d, const MConst &c, int32_t y, int dim, int pass, const double *pc,
+                                           const double *pn, const double *pp, const double *qn, const double *qp,
+                                           double xn, double xp, double tn, double tp, bool hn, bool hp,
+                                           const double *ts, double kap, double a_ip_in, double a_in_ip, double a_in_ia,
+                                           double *rhs, double &err) {
+    double sl[Sz<MODEL, D>::NE], sq;
+    sweep_axis_core<MODEL, D>(d, c, y, dim, pass, pc, pn, pp, qn, qp, xn, xp, tn, tp, hn, hp, ts, sl, sq);
+    // rhs[i] gains the dim-j terms in the reference's (i, j, k) order
+    if (pass == 0) rhs_add_dim<MODEL, D>(c, pc, sl, dim, kap, a_ip_in, a_in_ip, a_in_ia, rhs);
     err += sq;
 }
 
@@ -2446,6 +2459,96 @@ struct OpSweep {
         sweep_body<MODEL, D>(it.d, c, it.list, it.nl, li, it.L, it.dt, it.counter, pass, it.st, it.remesh);
     }
 };
+
+// The sweep of a cluster's listed cells (sweep_body) with a cell's axes on
+// D wavefronts of one block, as k_flux_axes: wavefront w fetches the cell's
+// two axis-w neighbours and computes the limited slope, gradient and error
+// term of axis w (slope and gradient stored in pass 0); the first wavefront
+// then adds the axes' rhs terms and error terms in axis order, exactly as
+// sweep_body's axis loop adds them, and takes the cell's decision.  Pass 1
+// re-sweeps the deferred cells only (nothing when none was deferred).
+template <int MODEL, int D>
+__global__ void __launch_bounds__(64 * D) k_sweep_axes(const Item *__restrict__ items, const MConst *__restrict__ mcs,
+                                                       const int *__restrict__ toff, int n_items, int tile_cells,
+                                                       int pass) {
+    using S = Sz<MODEL, D>;
+    constexpr int NE = S::NE;
+    __shared__ double SL[D - 1][NE][64];  // the limited slopes of axes 1 .. D-1
+    __shared__ double SQ[D - 1][64];      // their error terms
+    __shared__ uint8_t DEF[D][64];        // deferred by the axis
+    const int per_tile = tile_cells >> 6;
+    const int total = toff[n_items] * per_tile;
+    const int lane = (int) (threadIdx.x & 63), dim = (int) (threadIdx.x >> 6);
+    for (int g = (int) blockIdx.x; g < total; g += (int) gridDim.x) {
+        const int gg = xcd_block(g, total);
+        const int tile = gg / per_tile;
+        const int i = find_item(toff, n_items, tile);
+        const Item &it = items[i];
+        const MConst &c = mcs[i];
+        const Dev &d = it.d;
+        const int64_t C = d.C;
+        const int L = it.L;
+        const int li = (tile - toff[i]) * tile_cells + (gg % per_tile) * 64 + lane;
+        // an instance whose update produced a NaN stops (impl/integrate.hpp:50)
+        bool act = li < it.nl && !it.st->nan_flag && !(pass == 1 && it.st->n_deferred == 0);
+        const int32_t y = act ? it.list[li] : 0;
+        if (act && pass == 1 && !(d.mark[y] & MK_DEFER)) act = false;
+        if (!__syncthreads_or(act)) continue;  // uniform over the block
+        double pc[NE], ctr[D], ys = 0.0, yt = 0.0, kap = 0.0;
+        int ylev = 0;
+        double ts[S::NGX];
+#pragma unroll
+        for (int e = 0; e < S::NGX; ++e) ts[e] = 0.0;
+        bool deferred = false;
+        double sl[NE], sq = 0.0;
+        if (act) {
+            for (int e = 0; e < NE; ++e) pc[e] = d.p[e * C + y];
+            for (int a = 0; a < D; ++a) ctr[a] = d.center[a * C + y];
+            ys = d.size[y];
+            yt = d.t[y];
+            ylev = d.level[y];
+            if constexpr (MODEL == MULTI) kap = d.k[S::IKAPPA * C + y];
+            if constexpr (S::NG > 0)
+                for (int e = 0; e < S::NG; ++e) ts[e] = d.t_slope[e * C + y];
+            double pn[NE], pp[NE], qn[NE], qp[NE], xn, xp, tn, tp;
+            bool hn, hp;
+            sweep_side<MODEL, D, 0>(d, c, y, dim, pc, ctr, ys, yt, ylev, L, pass, pn, qn, tn, xn, hn, deferred);
+            sweep_side<MODEL, D, 1>(d, c, y, dim, pc, ctr, ys, yt, ylev, L, pass, pp, qp, tp, xp, hp, deferred);
+            sweep_axis_core<MODEL, D>(d, c, y, dim, pass, pc, pn, pp, qn, qp, xn, xp, tn, tp, hn, hp, ts, sl, sq);
+        }
+        if (dim > 0) {
+#pragma unroll
+            for (int e = 0; e < NE; ++e) SL[dim - 1][e][lane] = sl[e];
+            SQ[dim - 1][lane] = sq;
+        }
+        DEF[dim][lane] = deferred;
+        __syncthreads();
+        if (dim == 0 && act) {
+            double a_ip_in, a_in_ip, a_in_ia;
+            rhs_coeffs<MODEL, D>(c, pc, kap, a_ip_in, a_in_ip, a_in_ia);
+            double rhs[NE], err = 0.0;
+#pragma unroll
+            for (int e = 0; e < NE; ++e) rhs[e] = 0.0;
+            if (pass == 0) rhs_add_dim<MODEL, D>(c, pc, sl, 0, kap, a_ip_in, a_in_ip, a_in_ia, rhs);
+            err += sq;
+            bool def = deferred;
+#pragma unroll
+            for (int a = 1; a < D; ++a) {
+                double sa[NE];
+#pragma unroll
+                for (int e = 0; e < NE; ++e) sa[e] = SL[a - 1][e][lane];
+                if (pass == 0) rhs_add_dim<MODEL, D>(c, pc, sa, a, kap, a_ip_in, a_in_ip, a_in_ia, rhs);
+                err += SQ[a - 1][lane];
+                def = def || DEF[a][lane];
+            }
+            if (pass == 0)
+                for (int e = 0; e < NE; ++e) __builtin_nontemporal_store(rhs[e], &d.rhs[e * C + y]);
+            sweep_decide<MODEL, D>(d, c, y, pc, ctr, ys, ylev, L, it.dt, it.counter, def, err, it.st, it.remesh,
+                                   d.flags[y]);
+        }
+        __syncthreads();  // SL, SQ and DEF are reused by the next group
+    }
+}
 // the interface chain (impl/integrate.hpp:53-58) is skipped for an instance
 // whose update produced a NaN state
 template <int MODEL, int D>
