@@ -89,13 +89,14 @@ struct BOps {
             }
         }
         if constexpr (MODEL == MULTI) {
+            const unsigned ga = b->grid > 0 ? (unsigned) b->grid : std::max(1u, (unsigned) b->h_toff[n] * (unsigned) (B / 64));
             for (int r = 0; r < 3; ++r) {
-                k_batch<<<gl, B, 0, st>>>(it, b->d_mcs, b->d_toff, n, OpSmoothAlpha<MODEL, D>{});
+                k_axes<D><<<ga, 64 * D, 0, st>>>(it, b->d_mcs, b->d_toff, n, B, OpSmoothAlpha<MODEL, D>{});
                 k_batch<<<gl, B, 0, st>>>(it, b->d_mcs, b->d_toff, n, OpStoreK{S::IALPHAS});
             }
-            k_batch<<<gl, B, 0, st>>>(it, b->d_mcs, b->d_toff, n, OpNormal<MODEL, D>{});
-            k_batch<<<gl, B, 0, st>>>(it, b->d_mcs, b->d_toff, n, OpCurvature<MODEL, D>{});
-            k_batch<<<gl, B, 0, st>>>(it, b->d_mcs, b->d_toff, n, OpSmoothCurvature<MODEL, D>{});
+            k_axes<D><<<ga, 64 * D, 0, st>>>(it, b->d_mcs, b->d_toff, n, B, OpNormal<MODEL, D>{});
+            k_axes<D><<<ga, 64 * D, 0, st>>>(it, b->d_mcs, b->d_toff, n, B, OpCurvature<MODEL, D>{});
+            k_axes<D><<<ga, 64 * D, 0, st>>>(it, b->d_mcs, b->d_toff, n, B, OpSmoothCurvature<MODEL, D>{});
             k_batch<<<gl, B, 0, st>>>(it, b->d_mcs, b->d_toff, n, OpStoreK{S::IKAPPA});
         }
         // split readiness and unsplit detection of the marked cells

@@ -1099,27 +1099,25 @@ __device__ inline double bc_alpha(const Dev &d, const MConst &c, int32_t y, int 
     return pb[S::IA];
 }
 
-// smooth_alpha, one Jacobi sweep: impl/smooth.hpp:9-60
+// The interface chain's per-cell sums are one term per axis, added in axis
+// order; each op gives the term of one axis (axis) and the sum (finish), so
+// the axes can run on separate wavefronts (k_axes).
+// smooth_alpha, one Jacobi sweep (impl/smooth.hpp:9-60): the axis's term
+// 0.5 (a- + a+) + a
 template <int MODEL, int D>
-__device__ __forceinline__ void smooth_alpha_body(const Dev &d, const MConst &c, const int32_t *__restrict__ list, int nl, int li) {
+__device__ __forceinline__ double smooth_alpha_axis(const Dev &d, const MConst &c, int32_t y, int dim) {
     using S = Sz<MODEL, D>;
-    if (li >= nl) return;
     const int64_t C = d.C;
-    const int32_t y = list[li];
     const double ka = d.k[S::IALPHAS * C + y];
-    double acc = 0.0;
-    for (int dim = 0; dim < D; ++dim) {
-        const int32_t in = d.nbr[(2 * dim) * C + y], ip = d.nbr[(2 * dim + 1) * C + y];
-        double an, ap;
-        if (in == 0) an = bc_alpha<MODEL, D>(d, c, y, 2 * dim);
-        else if (!(d.flags[in] & F_LEAF)) an = interp_k<D>(d, in, S::IALPHAS);
-        else an = d.k[S::IALPHAS * C + in];
-        if (ip == 0) ap = bc_alpha<MODEL, D>(d, c, y, 2 * dim + 1);
-        else if (!(d.flags[ip] & F_LEAF)) ap = interp_k<D>(d, ip, S::IALPHAS);
-        else ap = d.k[S::IALPHAS * C + ip];
-        acc += 0.5 * (an + ap) + ka;
-    }
-    d.scratch[li] = acc / (2 * D);
+    const int32_t in = d.nbr[(2 * dim) * C + y], ip = d.nbr[(2 * dim + 1) * C + y];
+    double an, ap;
+    if (in == 0) an = bc_alpha<MODEL, D>(d, c, y, 2 * dim);
+    else if (!(d.flags[in] & F_LEAF)) an = interp_k<D>(d, in, S::IALPHAS);
+    else an = d.k[S::IALPHAS * C + in];
+    if (ip == 0) ap = bc_alpha<MODEL, D>(d, c, y, 2 * dim + 1);
+    else if (!(d.flags[ip] & F_LEAF)) ap = interp_k<D>(d, ip, S::IALPHAS);
+    else ap = d.k[S::IALPHAS * C + ip];
+    return 0.5 * (an + ap) + ka;
 }
 
 __device__ __forceinline__ void store_k_body(const Dev &d, const int32_t *__restrict__ list, int nl, int li, int kk) {
@@ -1128,115 +1126,99 @@ __device__ __forceinline__ void store_k_body(const Dev &d, const int32_t *__rest
 }
 
 // compute_normal: impl/normal_curvature.hpp:6-55
+// the axis's component of the alpha gradient, (a+ - a-) / (x+ - x-)
 template <int MODEL, int D>
-__device__ __forceinline__ void normal_body(const Dev &d, const MConst &c, const int32_t *__restrict__ list, int nl, int li) {
+__device__ __forceinline__ double normal_axis(const Dev &d, const MConst &c, int32_t y, int dim) {
     using S = Sz<MODEL, D>;
-    if (li >= nl) return;
     const int64_t C = d.C;
-    const int32_t y = list[li];
     const double ys = d.size[y];
-    double n[D];
-    for (int dim = 0; dim < D; ++dim) {
-        const int32_t in = d.nbr[(2 * dim) * C + y], ip = d.nbr[(2 * dim + 1) * C + y];
-        double xn, xp, an, ap;
-        if (in == 0) {
-            an = bc_alpha<MODEL, D>(d, c, y, 2 * dim);
-            xn = -0.5 * ys;
-        } else {
-            an = !(d.flags[in] & F_LEAF) ? interp_k<D>(d, in, S::IALPHAS) : d.k[S::IALPHAS * C + in];
-            xn = -0.5 * (ys + d.size[in]);
-        }
-        if (ip == 0) {
-            ap = bc_alpha<MODEL, D>(d, c, y, 2 * dim + 1);
-            xp = 0.5 * ys;
-        } else {
-            ap = !(d.flags[ip] & F_LEAF) ? interp_k<D>(d, ip, S::IALPHAS) : d.k[S::IALPHAS * C + ip];
-            xp = 0.5 * (ys + d.size[ip]);
-        }
-        n[dim] = (ap - an) / (xp - xn);
+    const int32_t in = d.nbr[(2 * dim) * C + y], ip = d.nbr[(2 * dim + 1) * C + y];
+    double xn, xp, an, ap;
+    if (in == 0) {
+        an = bc_alpha<MODEL, D>(d, c, y, 2 * dim);
+        xn = -0.5 * ys;
+    } else {
+        an = !(d.flags[in] & F_LEAF) ? interp_k<D>(d, in, S::IALPHAS) : d.k[S::IALPHAS * C + in];
+        xn = -0.5 * (ys + d.size[in]);
     }
-    double dd = 0;
-    for (int i = 0; i < D; ++i) dd += n[i] * n[i];
-    const double nrm = sqrt(dd);
-    for (int i = 0; i < D; ++i) d.k[(S::IKN + i) * C + y] = (nrm < 1e-10) ? 0.0 : n[i] / nrm;
+    if (ip == 0) {
+        ap = bc_alpha<MODEL, D>(d, c, y, 2 * dim + 1);
+        xp = 0.5 * ys;
+    } else {
+        ap = !(d.flags[ip] & F_LEAF) ? interp_k<D>(d, ip, S::IALPHAS) : d.k[S::IALPHAS * C + ip];
+        xp = 0.5 * (ys + d.size[ip]);
+    }
+    return (ap - an) / (xp - xn);
 }
 
 // compute_curvature: impl/normal_curvature.hpp:58-102
+// the axis's term of the curvature, (n+ - n-) / (x+ - x-) of the normal's
+// axis component (subtracted)
 template <int MODEL, int D>
-__device__ __forceinline__ void curvature_body(const Dev &d, const MConst &c, const int32_t *__restrict__ list, int nl, int li) {
+__device__ __forceinline__ double curvature_axis(const Dev &d, const MConst &c, int32_t y, int dim) {
     using S = Sz<MODEL, D>;
-    if (li >= nl) return;
     const int64_t C = d.C;
-    const int32_t y = list[li];
     const double ys = d.size[y];
-    double kappa = 0.0;
-    for (int dim = 0; dim < D; ++dim) {
-        const int32_t in = d.nbr[(2 * dim) * C + y], ip = d.nbr[(2 * dim + 1) * C + y];
-        double xn, xp, nn, np;
-        if (in == 0) {
-            nn = 0.0;
-            xn = -0.5 * ys;
-        } else {
-            nn = !(d.flags[in] & F_LEAF) ? interp_k<D>(d, in, S::IKN + dim) : d.k[(S::IKN + dim) * C + in];
-            xn = -0.5 * (ys + d.size[in]);
-        }
-        if (ip == 0) {
-            np = 0.0;
-            xp = 0.5 * ys;
-        } else {
-            np = !(d.flags[ip] & F_LEAF) ? interp_k<D>(d, ip, S::IKN + dim) : d.k[(S::IKN + dim) * C + ip];
-            xp = 0.5 * (ys + d.size[ip]);
-        }
-        kappa -= (np - nn) / (xp - xn);
+    const int32_t in = d.nbr[(2 * dim) * C + y], ip = d.nbr[(2 * dim + 1) * C + y];
+    double xn, xp, nn, np;
+    if (in == 0) {
+        nn = 0.0;
+        xn = -0.5 * ys;
+    } else {
+        nn = !(d.flags[in] & F_LEAF) ? interp_k<D>(d, in, S::IKN + dim) : d.k[(S::IKN + dim) * C + in];
+        xn = -0.5 * (ys + d.size[in]);
     }
-    d.k[S::IKAPPA * C + y] = kappa;
+    if (ip == 0) {
+        np = 0.0;
+        xp = 0.5 * ys;
+    } else {
+        np = !(d.flags[ip] & F_LEAF) ? interp_k<D>(d, ip, S::IKN + dim) : d.k[(S::IKN + dim) * C + ip];
+        xp = 0.5 * (ys + d.size[ip]);
+    }
+    return (np - nn) / (xp - xn);
 }
 
 // smooth_curvature (one iteration): impl/smooth.hpp:63-142
+// the axis's terms of the weighted sums: v[0] of curvature x weight, v[1]
+// of the weights
 template <int MODEL, int D>
-__device__ __forceinline__ void smooth_curvature_body(const Dev &d, const MConst &c, const int32_t *__restrict__ list, int nl, int li) {
+__device__ __forceinline__ void smooth_curvature_axis(const Dev &d, const MConst &c, int32_t y, int dim, double *v) {
     using S = Sz<MODEL, D>;
-    if (li >= nl) return;
     const int64_t C = d.C;
-    const int32_t y = list[li];
     const double kc = d.k[S::IKAPPA * C + y];
     const double al = d.k[S::IALPHAS * C + y];
     const double wa = al * (1 - al);
     const double wc = wa * wa;
-    double w = 0.0, ks = 0.0;
-    for (int dim = 0; dim < D; ++dim) {
-        const int32_t in = d.nbr[(2 * dim) * C + y], ip = d.nbr[(2 * dim + 1) * C + y];
-        double kn, kp, an, ap;
-        if (in == 0) {
-            kn = 0.0;
-            an = bc_alpha<MODEL, D>(d, c, y, 2 * dim);
-        } else if (!(d.flags[in] & F_LEAF)) {
-            double pi[S::NE];
-            kn = interp_k<D>(d, in, S::IKAPPA);
-            interp_p<MODEL, D>(d, in, pi);
-            an = pi[S::IA];
-        } else {
-            kn = d.k[S::IKAPPA * C + in];
-            an = d.p[S::IA * C + in];
-        }
-        if (ip == 0) {
-            kp = 0.0;
-            ap = bc_alpha<MODEL, D>(d, c, y, 2 * dim + 1);
-        } else if (!(d.flags[ip] & F_LEAF)) {
-            double pi[S::NE];
-            kp = interp_k<D>(d, ip, S::IKAPPA);
-            interp_p<MODEL, D>(d, ip, pi);
-            ap = pi[S::IA];
-        } else {
-            kp = d.k[S::IKAPPA * C + ip];
-            ap = d.p[S::IA * C + ip];
-        }
-        const double xa = an * (1 - an), xb = ap * (1 - ap);
-        const double wn = xa * xa, wp = xb * xb;
-        ks += 0.5 * (wn * kn + wp * kp) + kc * wc;
-        w += 0.5 * (wn + wp) + wc;
+    const int32_t in = d.nbr[(2 * dim) * C + y], ip = d.nbr[(2 * dim + 1) * C + y];
+    double kn, kp, an, ap;
+    if (in == 0) {
+        kn = 0.0;
+        an = bc_alpha<MODEL, D>(d, c, y, 2 * dim);
+    } else if (!(d.flags[in] & F_LEAF)) {
+        double pi[S::NE];
+        kn = interp_k<D>(d, in, S::IKAPPA);
+        interp_p<MODEL, D>(d, in, pi);
+        an = pi[S::IA];
+    } else {
+        kn = d.k[S::IKAPPA * C + in];
+        an = d.p[S::IA * C + in];
     }
-    d.scratch[li] = (w > 1e-6) ? ks / w : 0.0;
+    if (ip == 0) {
+        kp = 0.0;
+        ap = bc_alpha<MODEL, D>(d, c, y, 2 * dim + 1);
+    } else if (!(d.flags[ip] & F_LEAF)) {
+        double pi[S::NE];
+        kp = interp_k<D>(d, ip, S::IKAPPA);
+        interp_p<MODEL, D>(d, ip, pi);
+        ap = pi[S::IA];
+    } else {
+        kp = d.k[S::IKAPPA * C + ip];
+        ap = d.p[S::IA * C + ip];
+    }
+    const double xa = an * (1 - an), xb = ap * (1 - ap);
+    const double wn = xa * xa, wp = xb * xb;
+    v[0] = 0.5 * (wn * kn + wp * kp) + kc * wc;
+    v[1] = 0.5 * (wn + wp) + wc;
 }
 
 // --------------------------------------------------- refinement flagging
@@ -2551,11 +2533,20 @@ __global__ void __launch_bounds__(64 * D) k_sweep_axes(const Item *__restrict__ 
 }
 // the interface chain (impl/integrate.hpp:53-58) is skipped for an instance
 // whose update produced a NaN state
+// The interface chain's ops (impl/smooth.hpp, impl/normal_curvature.hpp)
+// as an axis term and an in-order sum, for k_axes; an instance whose update
+// produced a NaN state skips them
+// smooth_alpha (one iteration, impl/smooth.hpp:9-60): the mean into scratch
 template <int MODEL, int D>
 struct OpSmoothAlpha {
-    __device__ __forceinline__ void operator()(const Item &it, const MConst &c, int li) const {
-        if (it.st->nan_flag) return;
-        smooth_alpha_body<MODEL, D>(it.d, c, it.list, it.nl, li);
+    static constexpr int NV = 1;
+    __device__ __forceinline__ void axis(const Item &it, const MConst &c, int32_t y, int dim, double *v) const {
+        v[0] = smooth_alpha_axis<MODEL, D>(it.d, c, y, dim);
+    }
+    __device__ __forceinline__ void finish(const Item &it, const MConst &, int32_t, int li, const double (*w)[NV]) const {
+        double acc = 0.0;
+        for (int a = 0; a < D; ++a) acc += w[a][0];
+        it.d.scratch[li] = acc / (2 * D);
     }
 };
 struct OpStoreK {
@@ -2565,27 +2556,100 @@ struct OpStoreK {
         store_k_body(it.d, it.list, it.nl, li, kk);
     }
 };
+// compute_normal (impl/normal_curvature.hpp:6-55): the unit normal
 template <int MODEL, int D>
 struct OpNormal {
-    __device__ __forceinline__ void operator()(const Item &it, const MConst &c, int li) const {
-        if (it.st->nan_flag) return;
-        normal_body<MODEL, D>(it.d, c, it.list, it.nl, li);
+    static constexpr int NV = 1;
+    __device__ __forceinline__ void axis(const Item &it, const MConst &c, int32_t y, int dim, double *v) const {
+        v[0] = normal_axis<MODEL, D>(it.d, c, y, dim);
+    }
+    __device__ __forceinline__ void finish(const Item &it, const MConst &, int32_t y, int, const double (*w)[NV]) const {
+        using S = Sz<MODEL, D>;
+        double dd = 0;
+        for (int i = 0; i < D; ++i) dd += w[i][0] * w[i][0];
+        const double nrm = sqrt(dd);
+        for (int i = 0; i < D; ++i) it.d.k[(S::IKN + i) * it.d.C + y] = (nrm < 1e-10) ? 0.0 : w[i][0] / nrm;
     }
 };
+// compute_curvature (impl/normal_curvature.hpp:58-102): minus the normal's
+// divergence
 template <int MODEL, int D>
 struct OpCurvature {
-    __device__ __forceinline__ void operator()(const Item &it, const MConst &c, int li) const {
-        if (it.st->nan_flag) return;
-        curvature_body<MODEL, D>(it.d, c, it.list, it.nl, li);
+    static constexpr int NV = 1;
+    __device__ __forceinline__ void axis(const Item &it, const MConst &c, int32_t y, int dim, double *v) const {
+        v[0] = curvature_axis<MODEL, D>(it.d, c, y, dim);
+    }
+    __device__ __forceinline__ void finish(const Item &it, const MConst &, int32_t y, int, const double (*w)[NV]) const {
+        using S = Sz<MODEL, D>;
+        double kappa = 0.0;
+        for (int a = 0; a < D; ++a) kappa -= w[a][0];
+        it.d.k[S::IKAPPA * it.d.C + y] = kappa;
     }
 };
+// smooth_curvature (one iteration, impl/smooth.hpp:63-142): the weighted
+// mean into scratch
 template <int MODEL, int D>
 struct OpSmoothCurvature {
-    __device__ __forceinline__ void operator()(const Item &it, const MConst &c, int li) const {
-        if (it.st->nan_flag) return;
-        smooth_curvature_body<MODEL, D>(it.d, c, it.list, it.nl, li);
+    static constexpr int NV = 2;
+    __device__ __forceinline__ void axis(const Item &it, const MConst &c, int32_t y, int dim, double *v) const {
+        smooth_curvature_axis<MODEL, D>(it.d, c, y, dim, v);
+    }
+    __device__ __forceinline__ void finish(const Item &it, const MConst &, int32_t, int li, const double (*w)[NV]) const {
+        double ks = 0.0, wt = 0.0;
+        for (int a = 0; a < D; ++a) {
+            ks += w[a][0];
+            wt += w[a][1];
+        }
+        it.d.scratch[li] = (wt > 1e-6) ? ks / wt : 0.0;
     }
 };
+
+// A per-cell op whose result is a sum of one term per axis, added in axis
+// order (Op::axis gives axis w's NV terms, Op::finish adds them): a cell's
+// axes on D wavefronts of one block, 64 listed cells a block (as
+// k_flux_axes), the first wavefront adding the terms.  Skipped for an
+// instance whose update produced a NaN state.
+template <int D, class Op>
+__global__ void __launch_bounds__(64 * D) k_axes(const Item *__restrict__ items, const MConst *__restrict__ mcs,
+                                                 const int *__restrict__ toff, int n_items, int tile_cells, Op op) {
+    constexpr int NV = Op::NV;
+    __shared__ double V[D - 1][NV][64];
+    const int per_tile = tile_cells >> 6;
+    const int total = toff[n_items] * per_tile;
+    const int lane = (int) (threadIdx.x & 63), dim = (int) (threadIdx.x >> 6);
+    for (int g = (int) blockIdx.x; g < total; g += (int) gridDim.x) {
+        const int gg = xcd_block(g, total);
+        const int tile = gg / per_tile;
+        const int i = find_item(toff, n_items, tile);
+        const Item &it = items[i];
+        const MConst &c = mcs[i];
+        const int li = (tile - toff[i]) * tile_cells + (gg % per_tile) * 64 + lane;
+        const bool act = li < it.nl && !it.st->nan_flag;
+        if (!__syncthreads_or(act)) continue;  // uniform over the block
+        const int32_t y = act ? it.list[li] : 0;
+        double v[NV];
+#pragma unroll
+        for (int k = 0; k < NV; ++k) v[k] = 0.0;
+        if (act) op.axis(it, c, y, dim, v);
+        if (dim > 0) {
+#pragma unroll
+            for (int k = 0; k < NV; ++k) V[dim - 1][k][lane] = v[k];
+        }
+        __syncthreads();
+        if (dim == 0 && act) {
+            double w[D][NV];
+#pragma unroll
+            for (int k = 0; k < NV; ++k) w[0][k] = v[k];
+#pragma unroll
+            for (int a = 1; a < D; ++a)
+#pragma unroll
+                for (int k = 0; k < NV; ++k) w[a][k] = V[a - 1][k][lane];
+            op.finish(it, c, y, li, w);
+        }
+        __syncthreads();  // V is reused by the next group
+    }
+}
+
 struct OpApply {
     __device__ __forceinline__ void operator()(const Item &it, const MConst &c, int li) const {
         if (it.st->nan_flag) return;  // an instance with a NaN state stops before the LTS apply

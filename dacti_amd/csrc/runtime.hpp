@@ -285,13 +285,14 @@ struct Ops {
             HIPCHK(h, hipMemcpyAsync(b->d_toff, b->h_toff, 2 * sizeof(int), hipMemcpyHostToDevice, h->stream));
             HIPCHK(h, hipMemsetAsync(b->d_st, 0, sizeof(Status), h->stream));
             const unsigned g = std::max(1u, (unsigned) b->h_toff[1]);
+            const unsigned ga = g * (unsigned) (b->block / 64);  // k_axes: 64 cells a block
             for (int it = 0; it < 3; ++it) {
-                k_batch<<<g, b->block, 0, h->stream>>>(b->d_items, b->d_mcs, b->d_toff, 1, OpSmoothAlpha<MODEL, D>{});
+                k_axes<D><<<ga, 64 * D, 0, h->stream>>>(b->d_items, b->d_mcs, b->d_toff, 1, b->block, OpSmoothAlpha<MODEL, D>{});
                 k_batch<<<g, b->block, 0, h->stream>>>(b->d_items, b->d_mcs, b->d_toff, 1, OpStoreK{S::IALPHAS});
             }
-            k_batch<<<g, b->block, 0, h->stream>>>(b->d_items, b->d_mcs, b->d_toff, 1, OpNormal<MODEL, D>{});
-            k_batch<<<g, b->block, 0, h->stream>>>(b->d_items, b->d_mcs, b->d_toff, 1, OpCurvature<MODEL, D>{});
-            k_batch<<<g, b->block, 0, h->stream>>>(b->d_items, b->d_mcs, b->d_toff, 1, OpSmoothCurvature<MODEL, D>{});
+            k_axes<D><<<ga, 64 * D, 0, h->stream>>>(b->d_items, b->d_mcs, b->d_toff, 1, b->block, OpNormal<MODEL, D>{});
+            k_axes<D><<<ga, 64 * D, 0, h->stream>>>(b->d_items, b->d_mcs, b->d_toff, 1, b->block, OpCurvature<MODEL, D>{});
+            k_axes<D><<<ga, 64 * D, 0, h->stream>>>(b->d_items, b->d_mcs, b->d_toff, 1, b->block, OpSmoothCurvature<MODEL, D>{});
             k_batch<<<g, b->block, 0, h->stream>>>(b->d_items, b->d_mcs, b->d_toff, 1, OpStoreK{S::IKAPPA});
             HIPCHK(h, hipGetLastError());
             HIPCHK(h, hipStreamSynchronize(h->stream));
