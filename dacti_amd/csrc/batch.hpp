@@ -357,7 +357,8 @@ struct BOps {
         k_batch<<<list_grid(b), B, 0, st>>>(b->d_items, b->d_mcs, b->d_toff, n, OpUnsplit<MODEL, D>{});
         // neighbours, the rebuilt cluster list, its LTS decisions
         const unsigned gp = pool_grid(b);
-        k_pool_neighbors<D><<<gp, 256, 0, st>>>(b->d_items, b->d_mcs, b->d_toffp, n);
+        const unsigned gn = b->grid > 0 ? gp : gp * (unsigned) (POOL_TILE / 256 * 2 * D);  // a (node, face) a thread
+        k_pool_neighbors<D><<<gn, 256, 0, st>>>(b->d_items, b->d_mcs, b->d_toffp, n);
         k_sel_count<1><<<gp, 256, 0, st>>>(b->d_items, b->d_toffp, n, b->d_tilecnt);
         k_sel_scan<1><<<1, 1024, 0, st>>>(b->d_items, b->d_toffp, n, b->d_tilecnt);
         k_sel_scatter<1><<<gp, 256, 0, st>>>(b->d_items, b->d_toffp, n, b->d_tilecnt);

@@ -1490,32 +1490,36 @@ __device__ inline int32_t dom_find_from(const Dev &d, const MConst &c, double *p
 // for each face's neighbour starts at the face's previous neighbour (or, for
 // a new child, at its parent for an inner face and at the parent's neighbour
 // for an outer one): after a remesh it is the answer or one level off.
+// One face f of node i (the faces of a node are independent: each reads and
+// writes only its own link, and the parent's links are not written here).
 template <int D>
-__device__ __forceinline__ void neighbors_node(const Dev &d, const MConst &c, int64_t i, int64_t n) {
+__device__ __forceinline__ void neighbors_face(const Dev &d, const MConst &c, int64_t i, int f, int64_t n) {
     if (i < 1 || i >= n) return;
     const uint8_t fl = d.flags[i];
     if (!(fl & F_LEAF) || !(fl & F_ACTIVE) || (fl & F_GARBAGE)) return;
     const int64_t C = d.C;
+    if (d.bc[f * C + i]) return;
+    const int dim = f >> 1, dir = f & 1;
     const int lev = d.level[i];
     const double s = d.size[i];
     const int32_t par = d.parent[i];
-    for (int dim = 0; dim < D; ++dim)
-        for (int dir = 0; dir < 2; ++dir) {
-            const int f = 2 * dim + dir;
-            if (d.bc[f * C + i]) continue;
-            double loc[D];
-            for (int a = 0; a < D; ++a) loc[a] = d.center[a * C + i];
-            int32_t hint = d.nbr[f * C + i];
-            if (hint == 0 && par >= 1 && par < n) {
-                const bool inner = (dir == 1) == (loc[dim] < d.center[dim * C + par]);
-                hint = inner ? par : d.nbr[f * C + par];
-            }
-            loc[dim] = loc[dim] + s * (2 * dir - 1.0);
-            const int32_t nb = dom_find_from<D>(d, c, loc, lev, hint, n);
-            if (!nb || !(d.flags[nb] & F_ACTIVE)) continue;
-            d.nbr[f * C + i] = nb;
-            if (d.level[nb] < lev - 1) atomic_or_u8(&d.flags[nb], F_SHOULD_SPLIT);
-        }
+    double loc[D];
+    for (int a = 0; a < D; ++a) loc[a] = d.center[a * C + i];
+    int32_t hint = d.nbr[f * C + i];
+    if (hint == 0 && par >= 1 && par < n) {
+        const bool inner = (dir == 1) == (loc[dim] < d.center[dim * C + par]);
+        hint = inner ? par : d.nbr[f * C + par];
+    }
+    loc[dim] = loc[dim] + s * (2 * dir - 1.0);
+    const int32_t nb = dom_find_from<D>(d, c, loc, lev, hint, n);
+    if (!nb || !(d.flags[nb] & F_ACTIVE)) return;
+    d.nbr[f * C + i] = nb;
+    if (d.level[nb] < lev - 1) atomic_or_u8(&d.flags[nb], F_SHOULD_SPLIT);
+}
+
+template <int D>
+__device__ __forceinline__ void neighbors_node(const Dev &d, const MConst &c, int64_t i, int64_t n) {
+    for (int f = 0; f < 2 * D; ++f) neighbors_face<D>(d, c, i, f, n);
 }
 
 template <int D>
@@ -3249,19 +3253,23 @@ struct OpReassign {
     }
 };
 
+// The neighbour rebuild over the pool: one thread per (node, face), so a
+// thread's chain is one face search; POOL_TILE / 256 * 2D blocks a pool tile.
 template <int D>
 __global__ void __launch_bounds__(256) k_pool_neighbors(const Item *__restrict__ items,
                                                         const MConst *__restrict__ mcs, const int *__restrict__ toff,
                                                         int n_items) {
-    const int total = toff[n_items];
+    constexpr int PARTS = POOL_TILE / 256 * 2 * D;
+    const int total = toff[n_items] * PARTS;
     for (int t = blockIdx.x; t < total; t += gridDim.x) {
-        const int tile = xcd_block(t, total);
+        const int gg = xcd_block(t, total);
+        const int tile = gg / PARTS, part = gg % PARTS;
         const int i = find_item(toff, n_items, tile);
         const Item &it = items[i];
         // only where the topology changed (a split or an unsplit happened)
         if (!it.meshing || (it.st->n_ready_split == 0 && it.st->n_unsplit == 0)) continue;
-        const int64_t base = (int64_t) (tile - toff[i]) * POOL_TILE;
-        for (int k = 0; k < POOL_TILE; k += 256) neighbors_node<D>(it.d, mcs[i], base + k + threadIdx.x, it.n);
+        const int64_t node = (int64_t) (tile - toff[i]) * POOL_TILE + (part / (2 * D)) * 256 + threadIdx.x;
+        neighbors_face<D>(it.d, mcs[i], node, part % (2 * D), it.n);
     }
 }
 
