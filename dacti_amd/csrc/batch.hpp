@@ -73,7 +73,9 @@ struct BOps {
             k_flux_axes<MODEL, D><<<ga, 64 * D, 0, st>>>(it, b->d_mcs, b->d_toff, n, B);
             if (!fixed) {
                 k_batch<<<gl, B, 0, st>>>(it, b->d_mcs, b->d_toff, n, OpFluxRest<MODEL, D>{});
-                k_batch<<<gl, B, 0, st>>>(it, b->d_mcs, b->d_toff, n, OpDepositCoarse<MODEL, D>{});
+                // one instance: a (set-aside cell, face) a thread
+                if (n == 1) k_rest<<<gl, B, 0, st>>>(it, b->d_mcs, n, OpDepositCoarse<MODEL, D>{});
+                else k_batch<<<gl, B, 0, st>>>(it, b->d_mcs, b->d_toff, n, OpDepositCoarse<MODEL, D>{});
             }
         }
         k_batch<<<gl, B, 0, st>>>(it, b->d_mcs, b->d_toff, n, OpUpdate<MODEL, D>{});

@@ -640,35 +640,40 @@ __device__ void flux_cell(const Dev &d, const MConst &c, int32_t y, int L, doubl
 // the first computing sibling of the sub-face group adds the siblings'
 // fluxes (stored by flux_cell) to the receiver's face accumulator in
 // sub-face order.  Runs after every flux of the substep is computed.
+// One face f of y: the faces are independent (each writes only the
+// receiver's slot on that face, and one sibling of a group writes it).
 template <int MODEL, int D>
-__device__ void deposit_coarse(const Dev &d, int32_t y, int L) {
+__device__ void deposit_coarse_face(const Dev &d, int32_t y, int f, int L) {
     using S = Sz<MODEL, D>;
     constexpr int NF = S::NF;
     const int64_t C = d.C;
     const int ylev = d.level[y];
-#pragma unroll 1
-    for (int f = 0; f < 2 * D; ++f) {
-        const int dim = f >> 1, dir = f & 1;
-        const int32_t nb = d.nbr[f * C + y];
-        if (nb == 0 || !(d.flags[nb] & F_LEAF)) continue;
-        if (d.level[nb] >= ylev || d.acti[nb] > L || d.cluster[nb] == L) continue;
-        const int fo = 2 * dim + (1 - dir);
-        const int32_t rn = d.nbr[fo * C + nb];
-        int32_t first = 0;
-        for_sub_face_leaves<D>(d, rn, dim, dir, [&](int32_t x) {
-            if (first == 0 && d.cluster[x] == L && d.nbr[f * C + x] == nb) first = x;
-        });
-        if (first != y) continue;
-        for_sub_face_leaves<D>(d, rn, dim, dir, [&](int32_t x) {
-            if (d.cluster[x] == L && d.nbr[f * C + x] == nb) {
-                for (int e = 0; e < NF; ++e) {
-                    GMEM double &a = d.acc[(fo * NF + e) * C + nb];
-                    const double g = d.dep[(dim * NF + e) * C + x];
-                    a = dir == 1 ? a + g : a - g;
-                }
+    const int dim = f >> 1, dir = f & 1;
+    const int32_t nb = d.nbr[f * C + y];
+    if (nb == 0 || !(d.flags[nb] & F_LEAF)) return;
+    if (d.level[nb] >= ylev || d.acti[nb] > L || d.cluster[nb] == L) return;
+    const int fo = 2 * dim + (1 - dir);
+    const int32_t rn = d.nbr[fo * C + nb];
+    int32_t first = 0;
+    for_sub_face_leaves<D>(d, rn, dim, dir, [&](int32_t x) {
+        if (first == 0 && d.cluster[x] == L && d.nbr[f * C + x] == nb) first = x;
+    });
+    if (first != y) return;
+    for_sub_face_leaves<D>(d, rn, dim, dir, [&](int32_t x) {
+        if (d.cluster[x] == L && d.nbr[f * C + x] == nb) {
+            for (int e = 0; e < NF; ++e) {
+                GMEM double &a = d.acc[(fo * NF + e) * C + nb];
+                const double g = d.dep[(dim * NF + e) * C + x];
+                a = dir == 1 ? a + g : a - g;
             }
-        });
-    }
+        }
+    });
+}
+
+template <int MODEL, int D>
+__device__ void deposit_coarse(const Dev &d, int32_t y, int L) {
+#pragma unroll 1
+    for (int f = 0; f < 2 * D; ++f) deposit_coarse_face<MODEL, D>(d, y, f, L);
 }
 
 // ---------------------------------------------------------------- update
@@ -1701,15 +1706,16 @@ struct OpFluxRest {
 };
 template <int MODEL, int D>
 struct OpDepositCoarse {
-    __device__ __forceinline__ int count(const Item &it) const { return it.st->n_rest; }
+    // k_rest: a (set-aside cell, face) a thread
+    __device__ __forceinline__ int count(const Item &it) const { return it.st->n_rest * 2 * D; }
     __device__ __forceinline__ void cell(const Item &it, const MConst &, int r) const {
-        deposit_coarse<MODEL, D>(it.d, it.rest[r], it.L);
+        deposit_coarse_face<MODEL, D>(it.d, it.rest[r / (2 * D)], r % (2 * D), it.L);
     }
     __device__ __forceinline__ void operator()(const Item &it, const MConst &c, int li) const {
         const int B = (int) blockDim.x, T = (it.nl + B - 1) / B;
         const int r = (li % B) * T + li / B;
         if (r >= it.st->n_rest) return;
-        cell(it, c, r);
+        deposit_coarse<MODEL, D>(it.d, it.rest[r], it.L);
     }
 };
 
