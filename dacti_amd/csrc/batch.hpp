@@ -37,11 +37,12 @@ struct BOps {
 
     static int launch_head(dacti_batch *b) { return launch_head_ops(b, true); }
 
-    // the sweep with a cell's axes on separate wavefronts (k_sweep_axes) for
-    // lists of up to 1 M cells and on fixed grids (a captured sequence takes
-    // one form for every list); one thread per cell for longer lists, which
-    // fill the GPU without it (measured 2.5% faster on the 3D shock-drop)
-    static bool sweep_axes(const dacti_batch *b) {
+    // the sweep with a cell's axes on separate wavefronts (k_sweep_axes) and
+    // the coarse deposits with a (cell, face) a thread, for lists of up to
+    // 1 M cells and on fixed grids (a captured sequence takes one form for
+    // every list); one thread per cell for longer lists, which fill the GPU
+    // without it (the 3D shock-drop: sweep 2.5% faster overall, deposits 25%)
+    static bool split_small(const dacti_batch *b) {
         return b->grid > 0 || (int64_t) b->h_toff[b->n] * b->block < (int64_t(1) << 20);
     }
 
@@ -74,7 +75,7 @@ struct BOps {
             if (!fixed) {
                 k_batch<<<gl, B, 0, st>>>(it, b->d_mcs, b->d_toff, n, OpFluxRest<MODEL, D>{});
                 // one instance: a (set-aside cell, face) a thread
-                if (n == 1) k_rest<<<gl, B, 0, st>>>(it, b->d_mcs, n, OpDepositCoarse<MODEL, D>{});
+                if (n == 1 && split_small(b)) k_rest<<<gl, B, 0, st>>>(it, b->d_mcs, n, OpDepositCoarse<MODEL, D>{});
                 else k_batch<<<gl, B, 0, st>>>(it, b->d_mcs, b->d_toff, n, OpDepositCoarse<MODEL, D>{});
             }
         }
@@ -83,7 +84,7 @@ struct BOps {
             k_sweep_window<MODEL, D><<<window_grid(b), 64, 0, st>>>(it, b->d_mcs);
             if (!fixed) k_rest<<<gr, B, 0, st>>>(it, b->d_mcs, n, OpSweepRest<MODEL, D>{});
         } else {
-            if (sweep_axes(b)) {
+            if (split_small(b)) {
                 const unsigned ga = b->grid > 0 ? (unsigned) b->grid : std::max(1u, (unsigned) b->h_toff[n] * (unsigned) (B / 64));
                 k_sweep_axes<MODEL, D><<<ga, 64 * D, 0, st>>>(it, b->d_mcs, b->d_toff, n, B, 0);
             } else {
@@ -461,7 +462,7 @@ struct BOps {
                 if (rc) return rc;
             }
             if (deferred) {
-                if (sweep_axes(b)) {
+                if (split_small(b)) {
                     const unsigned ga = b->grid > 0 ? (unsigned) b->grid
                                                     : std::max(1u, (unsigned) b->h_toff[n] * (unsigned) (b->block / 64));
                     k_sweep_axes<MODEL, D><<<ga, 64 * D, 0, b->stream>>>(b->d_items, b->d_mcs, b->d_toff, n, b->block, 1);
