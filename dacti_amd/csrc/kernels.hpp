@@ -1720,9 +1720,11 @@ struct OpDepositCoarse {
 };
 
 // An op over the set-aside cells of each item with a list, r = 0 ..
-// op.count(item) - 1, spread round-robin over the blocks.  For a single
-// instance next to the window ops: the grid does not stride over the list's
-// tiles (a dense cluster has tens of thousands) to find the few cells.
+// op.count(item) - 1 (a cell, or a (cell, face) pair), spread round-robin
+// over the blocks.  For a single instance (each block reads every item's
+// count): next to the window ops the grid does not stride over the list's
+// tiles (a dense cluster has tens of thousands) to find the few cells, and
+// the coarse deposits of small lists take a thread per (cell, face).
 template <class Op>
 __global__ void __launch_bounds__(256) k_rest(const Item *__restrict__ items, const MConst *__restrict__ mcs,
                                               int n_items, Op op) {
