@@ -160,7 +160,7 @@ struct BOps {
         k_rn_count<<<G, 256, 0, st>>>(it, b->d_toffr, n, b->d_tilecnt);
         k_sel_scan<2><<<1, 1024, 0, st>>>(it, b->d_toffr, n, b->d_tilecnt);
         k_rn_scatter<<<G, 256, 0, st>>>(it, b->d_toffr, n, b->d_tilecnt);
-        k_batch<<<G, B, 0, st>>>(it, b->d_mcs, b->d_toff, n, OpSplitSlopes<MODEL, D>{});
+        k_axes<D><<<G, 64 * D, 0, st>>>(it, b->d_mcs, b->d_toff, n, B, OpSplitSlopesAxes<MODEL, D>{});
         k_batch<<<G, B, 0, st>>>(it, b->d_mcs, b->d_toff, n, OpSplit<MODEL, D>{});
         k_batch<<<G, B, 0, st>>>(it, b->d_mcs, b->d_toff, n, OpUnsplit<MODEL, D>{});
         k_pool_neighbors<D><<<G, 256, 0, st>>>(it, b->d_mcs, b->d_toffp, n);
@@ -355,7 +355,8 @@ struct BOps {
         k_rn_count<<<gr, 256, 0, st>>>(b->d_items, b->d_toffr, n, b->d_tilecnt);
         k_sel_scan<2><<<1, 1024, 0, st>>>(b->d_items, b->d_toffr, n, b->d_tilecnt);
         k_rn_scatter<<<gr, 256, 0, st>>>(b->d_items, b->d_toffr, n, b->d_tilecnt);
-        k_batch<<<list_grid(b), B, 0, st>>>(b->d_items, b->d_mcs, b->d_toff, n, OpSplitSlopes<MODEL, D>{});
+        const unsigned gs = b->grid > 0 ? (unsigned) b->grid : std::max(1u, (unsigned) b->h_toff[n] * (unsigned) (B / 64));
+        k_axes<D><<<gs, 64 * D, 0, st>>>(b->d_items, b->d_mcs, b->d_toff, n, B, OpSplitSlopesAxes<MODEL, D>{});
         k_batch<<<list_grid(b), B, 0, st>>>(b->d_items, b->d_mcs, b->d_toff, n, OpSplit<MODEL, D>{});
         k_batch<<<list_grid(b), B, 0, st>>>(b->d_items, b->d_mcs, b->d_toff, n, OpUnsplit<MODEL, D>{});
         // neighbours, the rebuilt cluster list, its LTS decisions

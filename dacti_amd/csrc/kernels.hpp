@@ -743,19 +743,20 @@ __device__ __forceinline__ void update_body(const Dev &d, const MConst &c, const
 
 // ----------------------------------------------------------------- slope
 // compute_slope: impl/slopes.hpp:15-76
+// compute_slope's axis dim (slopes.hpp:15-76): the limited slope (stored and
+// returned in sl) and the gradient of that axis; the axes are independent
 template <int MODEL, int D>
-__device__ inline void compute_slope_dev(const Dev &d, const MConst &c, int32_t y) {
+__device__ inline void compute_slope_axis(const Dev &d, const MConst &c, int32_t y, int dim, double *sl) {
     using S = Sz<MODEL, D>;
     constexpr int NE = S::NE;
     const int64_t C = d.C;
-    double pc[NE], slope[D][NE];
+    double pc[NE];
     for (int e = 0; e < NE; ++e) pc[e] = d.p[e * C + y];
     const double ys = d.size[y], yt = d.t[y];
     double ts[S::NGX];
     if constexpr (S::NG > 0)
         for (int e = 0; e < S::NG; ++e) ts[e] = d.t_slope[e * C + y];
-#pragma unroll
-    for (int dim = 0; dim < D; ++dim) {
+    {
         const int32_t in = d.nbr[(2 * dim) * C + y], ip = d.nbr[(2 * dim + 1) * C + y];
         double pn[NE], pp[NE], xn, xp, tn, tp;
         const double xc = 0.0;
@@ -798,8 +799,8 @@ __device__ inline void compute_slope_dev(const Dev &d, const MConst &c, int32_t 
         for (int e = 0; e < NE; ++e) {
             const double sn = (pc[e] - pn[e]) / (xc - xn);
             const double sp = (pp[e] - pc[e]) / (xp - xc);
-            slope[dim][e] = limiter(c.limiter, sn, sp);
-            d.slope[(dim * NE + e) * C + y] = slope[dim][e];
+            sl[e] = limiter(c.limiter, sn, sp);
+            d.slope[(dim * NE + e) * C + y] = sl[e];
         }
         if constexpr (S::NG > 0) {
             double gp[S::NG], gn[S::NG];
@@ -809,6 +810,17 @@ __device__ inline void compute_slope_dev(const Dev &d, const MConst &c, int32_t 
                 d.grad[(dim * S::NGX + e) * C + y] = (gp[e] - gn[e] - ts[e] * (tp - tn)) / (xp - xn);
         }
     }
+}
+
+template <int MODEL, int D>
+__device__ inline void compute_slope_dev(const Dev &d, const MConst &c, int32_t y) {
+    using S = Sz<MODEL, D>;
+    constexpr int NE = S::NE;
+    const int64_t C = d.C;
+    double pc[NE], slope[D][NE];
+#pragma unroll
+    for (int dim = 0; dim < D; ++dim) compute_slope_axis<MODEL, D>(d, c, y, dim, slope[dim]);
+    for (int e = 0; e < NE; ++e) pc[e] = d.p[e * C + y];
     double rhs[NE];
     const double kap = MODEL == MULTI ? d.k[S::IKAPPA * C + y] : 0.0;
     compute_rhs<MODEL, D>(c, pc, slope, kap, rhs);
@@ -1262,14 +1274,6 @@ __device__ __forceinline__ void mark2_body(const Dev &d, const int32_t *__restri
     ready_new[li] = rn;
 }
 
-// slopes of the cells about to split, on the pre-split tree (:169)
-template <int MODEL, int D>
-__device__ __forceinline__ void split_slopes_body(const Dev &d, const MConst &c, const int32_t *__restrict__ list,
-                                                  int nl, int li) {
-    if (li >= nl) return;
-    const int32_t y = list[li];
-    if (d.mark[y] & MK_READY) compute_slope_dev<MODEL, D>(d, c, y);
-}
 
 // Domain::split (domain.hpp:192-247) + child set-up (:174-195).  new_off is
 // the exclusive scan of "needs a new child block" over the list.
@@ -2552,6 +2556,7 @@ __global__ void __launch_bounds__(64 * D) k_sweep_axes(const Item *__restrict__ 
 template <int MODEL, int D>
 struct OpSmoothAlpha {
     static constexpr int NV = 1;
+    __device__ __forceinline__ bool take(const Item &, int32_t) const { return true; }
     __device__ __forceinline__ void axis(const Item &it, const MConst &c, int32_t y, int dim, double *v) const {
         v[0] = smooth_alpha_axis<MODEL, D>(it.d, c, y, dim);
     }
@@ -2572,6 +2577,7 @@ struct OpStoreK {
 template <int MODEL, int D>
 struct OpNormal {
     static constexpr int NV = 1;
+    __device__ __forceinline__ bool take(const Item &, int32_t) const { return true; }
     __device__ __forceinline__ void axis(const Item &it, const MConst &c, int32_t y, int dim, double *v) const {
         v[0] = normal_axis<MODEL, D>(it.d, c, y, dim);
     }
@@ -2588,6 +2594,7 @@ struct OpNormal {
 template <int MODEL, int D>
 struct OpCurvature {
     static constexpr int NV = 1;
+    __device__ __forceinline__ bool take(const Item &, int32_t) const { return true; }
     __device__ __forceinline__ void axis(const Item &it, const MConst &c, int32_t y, int dim, double *v) const {
         v[0] = curvature_axis<MODEL, D>(it.d, c, y, dim);
     }
@@ -2603,6 +2610,7 @@ struct OpCurvature {
 template <int MODEL, int D>
 struct OpSmoothCurvature {
     static constexpr int NV = 2;
+    __device__ __forceinline__ bool take(const Item &, int32_t) const { return true; }
     __device__ __forceinline__ void axis(const Item &it, const MConst &c, int32_t y, int dim, double *v) const {
         smooth_curvature_axis<MODEL, D>(it.d, c, y, dim, v);
     }
@@ -2613,6 +2621,31 @@ struct OpSmoothCurvature {
             wt += w[a][1];
         }
         it.d.scratch[li] = (wt > 1e-6) ? ks / wt : 0.0;
+    }
+};
+
+// The slopes (and rhs) of the cells about to split, on the pre-split tree
+// (adaptive_refinement.hpp:169): compute_slope per axis, the rhs from the
+// axes' slopes in axis order (as compute_slope_dev)
+template <int MODEL, int D>
+struct OpSplitSlopesAxes {
+    static constexpr int NV = Sz<MODEL, D>::NE;
+    __device__ __forceinline__ bool take(const Item &it, int32_t y) const {
+        return it.meshing && (it.d.mark[y] & MK_READY);
+    }
+    __device__ __forceinline__ void axis(const Item &it, const MConst &c, int32_t y, int dim, double *v) const {
+        compute_slope_axis<MODEL, D>(it.d, c, y, dim, v);
+    }
+    __device__ __forceinline__ void finish(const Item &it, const MConst &c, int32_t y, int, const double (*w)[NV]) const {
+        using S = Sz<MODEL, D>;
+        constexpr int NE = S::NE;
+        const Dev &d = it.d;
+        const int64_t C = d.C;
+        double pc[NE], rhs[NE];
+        for (int e = 0; e < NE; ++e) pc[e] = d.p[e * C + y];
+        const double kap = MODEL == MULTI ? d.k[S::IKAPPA * C + y] : 0.0;
+        compute_rhs<MODEL, D>(c, pc, w, kap, rhs);
+        for (int e = 0; e < NE; ++e) d.rhs[e * C + y] = rhs[e];
     }
 };
 
@@ -2636,9 +2669,10 @@ __global__ void __launch_bounds__(64 * D) k_axes(const Item *__restrict__ items,
         const Item &it = items[i];
         const MConst &c = mcs[i];
         const int li = (tile - toff[i]) * tile_cells + (gg % per_tile) * 64 + lane;
-        const bool act = li < it.nl && !it.st->nan_flag;
+        const bool listed = li < it.nl && !it.st->nan_flag;
+        const int32_t y = listed ? it.list[li] : 0;
+        const bool act = listed && op.take(it, y);
         if (!__syncthreads_or(act)) continue;  // uniform over the block
-        const int32_t y = act ? it.list[li] : 0;
         double v[NV];
 #pragma unroll
         for (int k = 0; k < NV; ++k) v[k] = 0.0;
@@ -3231,13 +3265,6 @@ static __global__ void __launch_bounds__(256) k_rn_scatter(const Item *__restric
     }
 }
 
-template <int MODEL, int D>
-struct OpSplitSlopes {
-    __device__ __forceinline__ void operator()(const Item &it, const MConst &c, int li) const {
-        if (!it.meshing) return;
-        split_slopes_body<MODEL, D>(it.d, c, it.list, it.nl, li);
-    }
-};
 template <int MODEL, int D>
 struct OpSplit {
     __device__ __forceinline__ void operator()(const Item &it, const MConst &c, int li) const {
